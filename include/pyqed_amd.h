@@ -1,0 +1,181 @@
+/*
+ * pyqed_amd.h -- C ABI of the MI355X (gfx950) open-system propagator.
+ *
+ * The library (libpyqed_amd.so) advances HEOM hierarchies and Liouville-space
+ * master equations (Lindblad, Redfield) with hand-written HIP kernels.  Python
+ * callers (pyqed_amd.heom.deom.DEOMSolver, pyqed_amd.oqs.LindbladSolver /
+ * RedfieldSolver) bind it with ctypes; any FFI that can pass pointers works.
+ *
+ * Conventions
+ *  - Every entry point returns an int status: PYQED_OK (0) or an error code;
+ *    pyqed_last_error() returns a thread-local message for the last failure.
+ *  - Complex numbers are interleaved (re, im) pairs of double; matrices are
+ *    row-major, element (i, j) of a d x d matrix at index i*d + j.  This is the
+ *    index order of the reference's dense generator (pyqed/heom/deom.py:769-771).
+ *  - Pointers documented as "device" point to GPU memory owned by the caller
+ *    (e.g. torch tensors); "host" pointers are ordinary CPU memory.  The library
+ *    owns only opaque plans and the device tables inside them.
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream).  Run
+ *    calls are stream-ordered, do not synchronise the host, and are safe to
+ *    call concurrently on distinct plans.
+ */
+#ifndef PYQED_AMD_H
+#define PYQED_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYQED_OK 0
+#define PYQED_EINVAL 1       /* bad argument (shape, size, null pointer, unsupported value) */
+#define PYQED_ENOMEM 2       /* device or host allocation failed */
+#define PYQED_EHIP 3         /* a HIP runtime call failed */
+#define PYQED_EUNSUPPORTED 4 /* valid input that this build does not handle */
+
+#define PYQED_PREC_FP64 0  /* plain FP64: right-hand side, stage inputs and state all FP64 */
+#define PYQED_PREC_MIXED 1 /* FP32 right-hand side and stage inputs, FP64 state and RK4 accumulator */
+
+/* Plan flags */
+#define PYQED_PLAN_NO_RESIDENT 1 /* never run instances LDS-resident (always stream through HBM) */
+#define PYQED_PLAN_DENSE_Q 2     /* coupling operators given as dense d x d matrices (qmat) */
+
+/* Thread-local description of the last error (never NULL). */
+const char* pyqed_last_error(void);
+/* Library version as 10000*major + 100*minor + patch. */
+int pyqed_version(void);
+/* Number of HIP devices visible; 0 when none (status PYQED_OK either way). */
+int pyqed_device_count(int* count);
+
+/* ------------------------------------------------------------------ HEOM
+ * One HEOM instance: system Hamiltonian, bath exponents and couplings.
+ * Mirrors the state DEOMSolver builds in check_()/init_() and the arguments
+ * of rk4()/generate_dot_element() (pyqed/heom/deom.py:641-673, 953-1070):
+ *   dRho_n/dt = -(sum_k n_k expn_k) Rho_n - i[H(t), Rho_n]
+ *     - i sum_{k: n_k>0} sqrt(n_k)/sqrt(etaa_k) (etal_k Q Rho_{n-e_k} - etar_k Rho_{n-e_k} Q)
+ *     - i sum_{k: |n|<L} sqrt(n_k+1) sqrt(etaa_k) [Q, Rho_{n+e_k}],   Q = Q_{mode[k]}(t)
+ * with H(t) = H + sdip f_s(t) and Q_m(t) = Q_m + cdip_m f_c(t) (deom.py:676-687).
+ * All pointers are host pointers; the plan copies what it needs. */
+typedef struct {
+  int32_t d;              /* system dimension (nsys), 1..32 */
+  int32_t K;              /* number of bath exponentials (nind), 0..255 */
+  int32_t L;              /* hierarchy depth (lmax), 0..127 */
+  int32_t ncoup;          /* number of coupling operators Q_m */
+  const int32_t* mode;    /* [K] coupling index of each exponential (Bath.mode) */
+  const double* expn;     /* [K] complex gamma_k (Bath.expn) */
+  const double* etal;     /* [K] complex (Bath.etal) */
+  const double* etar;     /* [K] complex (Bath.etar) */
+  const double* etaa;     /* [K] complex (Bath.etaa) */
+  const double* H;        /* [d*d] complex system Hamiltonian */
+  const double* sdip;     /* [d*d] complex system dipole, or NULL */
+  const double* qdiag;    /* [ncoup*d] complex diagonals of Q_m (when not PYQED_PLAN_DENSE_Q) */
+  const double* cdiag;    /* [ncoup*d] complex diagonals of the coupling dipoles, or NULL */
+  const double* qmat;     /* [ncoup*d*d] complex dense Q_m (with PYQED_PLAN_DENSE_Q), else NULL */
+  const double* cmat;     /* [ncoup*d*d] complex dense coupling dipoles (with PYQED_PLAN_DENSE_Q), or NULL */
+} pyqed_heom_instance;
+
+typedef struct pyqed_heom_plan pyqed_heom_plan;
+
+/* Build the hierarchy tables (keys, n+e_k and n-e_k ranks: deom.py:555-638
+ * and init_() deom.py:1048-1064) for each instance in host C++, pack every
+ * instance into one state layout and upload tables and coefficients.
+ * Replaces DEOMSolver.check_()/init_() (deom.py:1035-1064). */
+int pyqed_heom_plan_create(const pyqed_heom_instance* inst, int32_t n_inst, int32_t flags,
+                           pyqed_heom_plan** plan);
+int pyqed_heom_plan_destroy(pyqed_heom_plan* plan);
+
+/* Sizes of the packed state: total complex elements (sum nmax*d*d), total
+ * ADOs, and the workspace bytes run() needs for each precision. */
+int pyqed_heom_plan_query(const pyqed_heom_plan* plan, int64_t* n_elems, int64_t* n_ados,
+                          int64_t* ws_bytes_fp64, int64_t* ws_bytes_mixed);
+/* Per instance: element offset of ADO 0 in the packed state, nmax, and the
+ * offset of its d*d block inside one trajectory slot (host arrays [n_inst]). */
+int pyqed_heom_plan_layout(const pyqed_heom_plan* plan, int64_t* state_offset, int64_t* nmax,
+                           int64_t* traj_offset);
+/* Copy instance `inst`'s tables to host arrays of shape [nmax][K]: keys
+ * (int64, as DEOMSolver.keys), plus (rank of n+e_k, -1 at tier L) and minus
+ * (rank of n-e_k, -1 where n_k = 0) -- what hash_plus / hash_minus return
+ * (deom.py:588-605).  Any pointer may be NULL. */
+int pyqed_heom_tables_export(const pyqed_heom_plan* plan, int32_t inst, int64_t* keys, int32_t* plus,
+                             int32_t* minus);
+
+/* Host-only (no GPU needed): number of ADOs C(K+L, L) of a (K, L) hierarchy,
+ * and its tables in the layout of pyqed_heom_tables_export.  Built by the same
+ * C++ code the plans use.  Replaces DEOMSolver.init_() + gen_keys
+ * (deom.py:609-638, 1048-1064) for inspection. */
+int pyqed_heom_tables_size(int32_t K, int32_t L, int64_t* nmax);
+int pyqed_heom_tables_build(int32_t K, int32_t L, int64_t* keys, int32_t* plus, int32_t* minus);
+
+/* Advance every instance of the plan by `nsteps` classic RK4 steps of size
+ * dt, starting at global step index `step0` (time step0*dt).
+ * Replaces the time loop of DEOMSolver.run and rk4() (deom.py:725-766,
+ * 1072-1114).
+ *   y          device, [n_elems] complex double: the packed hierarchy state,
+ *              updated in place (ADO 0 of each instance is the reduced rho).
+ *   workspace  device, >= ws_bytes for `precision` (plan_query); scratch.
+ *   field      device [(step0+nsteps)*6] double or NULL: for step s,
+ *              field[6s+0..2] = f_s(t), f_s(t+dt/2), f_s(t+dt) and
+ *              field[6s+3..5] = f_c at the same times (t = s*dt).
+ *   traj       device or NULL: after every `save_every` steps, the rho of
+ *              every instance is written to slot j (j = 0, 1, ...) at
+ *              traj[j*traj_stride + traj_offset[i]], complex double, where
+ *              traj_stride = sum of d*d over instances (complex elements).
+ *   stream     hipStream_t. */
+int pyqed_heom_run(pyqed_heom_plan* plan, int32_t precision, void* y, void* workspace, int64_t ws_bytes,
+                   const double* field, int64_t step0, int64_t nsteps, double dt, void* traj,
+                   int64_t save_every, void* stream);
+
+/* Evaluate the right-hand side once: out = RHS(x) at time index `step`
+ * stage column `col` (0: t, 1: t+dt/2, 2: t+dt) in FP64 (device pointers,
+ * [n_elems] complex double).  Replaces rem_cal (deom.py:667-673); used to
+ * check the operator against the dense generator. */
+int pyqed_heom_rhs(pyqed_heom_plan* plan, const void* x, void* out, const double* field, int64_t step,
+                   int32_t col, void* stream);
+
+/* Kernel-level statistics of the last run (for the benchmark): number of
+ * streamed work items, resident instances, streamed elements, resident
+ * elements. */
+int pyqed_heom_plan_stats(const pyqed_heom_plan* plan, int64_t* stats4);
+
+/* ------------------------------------------------------------------ Liouville-space RK4
+ * Batched propagation of vec(rho) (row-major, length N = d*d) under
+ *   d vec(rho)/dt = (A_0 + sum_{i=1..nterm-1} f_i(step) A_i) vec(rho),
+ * with the field constant over each RK4 step, using the RK4 of
+ * pyqed/phys.py:1051-1063.  Lindblad (oqs.py:697-714, 1596-1806) and
+ * Redfield (oqs.py:364-459, 519-570) both map onto it: the host builds the
+ * generators A_i (N x N complex, row-major) once.
+ *   A        device [nterm][N][N] complex double (shared by all instances)
+ *   rho      device [n_inst][N] complex double, advanced in place
+ *   field    device [n_inst][nsteps][nterm-1] double, or NULL when nterm == 1
+ *   obs      device [n_obs][N] complex double: observable Tr(O rho) is
+ *            computed as sum_e obs[o][e] * rho[e]; pass O^T flattened so that
+ *            the sum is Tr(O rho) (may be NULL when n_obs == 0)
+ *   obs_out  device [n_inst][nsteps+1][n_obs] complex double (row 0 = t0)
+ *   rho_out  device [n_inst][nsteps/save_every][N] complex double, or NULL */
+int pyqed_liouville_run(int32_t N, int32_t nterm, const void* A, int32_t n_inst, void* rho,
+                        const double* field, int64_t nsteps, double dt, int32_t n_obs, const void* obs,
+                        void* obs_out, void* rho_out, int64_t save_every, void* stream);
+
+/* Adjoint of pyqed_liouville_run for a real objective J(rho(T)) (no
+ * reference counterpart: pyqed has no gradients).  Forward states are
+ * checkpointed every `ckpt_every` steps and each segment is recomputed
+ * before it is swept backwards (discrete adjoint of the RK4 step).
+ *   rhoT_bar  device [n_inst][N] complex double: dJ/dRe rho(T) + i dJ/dIm rho(T)
+ *   grad_field device [n_inst][nsteps][nterm-1] double: dJ/df_i(step)
+ *   grad_A     device [n_inst][nterm][N][N] complex double or NULL:
+ *              sum over steps of the adjoint outer products, so that
+ *              dJ/dtheta = Re sum conj(grad_A) * dA/dtheta for any generator
+ *              parameter theta (Hamiltonian energies, rates)
+ *   rho0_bar   device [n_inst][N] complex double or NULL: gradient w.r.t. rho(0)
+ *   ckpt       device workspace, >= n_inst*(nsteps/ckpt_every+1)*N*16 bytes */
+int pyqed_liouville_adjoint(int32_t N, int32_t nterm, const void* A, int32_t n_inst, const void* rho0,
+                            const double* field, int64_t nsteps, double dt, int32_t ckpt_every,
+                            const void* rhoT_bar, double* grad_field, void* grad_A, void* rho0_bar,
+                            void* ckpt, int64_t ckpt_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYQED_AMD_H */

@@ -1,0 +1,985 @@
+// HEOM propagator for gfx950: host-side plan (hierarchy tables, packed
+// layout, work queue) and the fused RK4-stage kernels.
+//
+// Operator (pyqed/heom/deom.py:641-673, generate_dot_element / rem_cal):
+//   dRho_n/dt = -(sum_k n_k g_k) Rho_n - i[H(t), Rho_n]
+//     + sum_{k: n_k>0}  sqrt(n_k)   (cLm_k Q Rho_{n-e_k} + cRm_k Rho_{n-e_k} Q)
+//     + sum_{k: |n|<L}  sqrt(n_k+1) (cLp_k Q Rho_{n+e_k} + cRp_k Rho_{n+e_k} Q)
+// with cLm = -i etal/sqrt(etaa), cRm = i etar/sqrt(etaa), cLp = -i sqrt(etaa),
+// cRp = i sqrt(etaa) and Q = Q_{mode[k]}(t).
+//
+// Time stepping (deom.py:725-766) is classic RK4.  Each stage is one kernel
+// launch that evaluates the right-hand side of every ADO and applies that
+// stage's update, so the state crosses HBM once per stage.  The accumulator
+// register holds the running new state (ynew = y + dt/6 k1 + dt/3 k2 + ...),
+// which lets the last stage write y without re-reading it:
+//   stage 0: k1 = F(y);  ynew  = y + k1 dt/6;  X = y + k1 dt/2
+//   stage 1: k2 = F(X);  ynew += k2 dt/3;      X = y + k2 dt/2
+//   stage 2: k3 = F(X);  ynew += k3 dt/3;      X = y + k3 dt
+//   stage 3: k4 = F(X);  y     = ynew + k4 dt/6
+// Right-hand-side arithmetic runs in T (double for the plain FP64 path,
+// float for the mixed path); y and ynew are always double.
+//
+// Layout in HBM: instances packed one after another; inside an instance ADO
+// a (rank in the reference's combinadic order) occupies elements
+// [a*d*d, (a+1)*d*d), row-major (deom.py:769-771).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+using namespace pqa;
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxD = 32;
+constexpr int kMaxK = 255;
+constexpr int kMaxL = 126;
+
+// ------------------------------------------------------------------ device structures
+struct InstDev {
+  int32_t d, d2, K, L, ncoup, nmax;
+  int32_t flags;  // bit0: sdip present, bit1: coupling dipole present, bit2: dense Q
+  int32_t grp;    // lanes cooperating on one ADO in the streamed kernel
+  int64_t off;    // element offset of ADO 0 in the packed state
+  int64_t traj_off;
+  int32_t rowptr_off, ent_off, coef_off, mode_off;
+  int32_t h_off, q_off;  // offsets (complex elements) into the matrix pool
+};
+
+struct WorkItem {
+  int32_t inst, ado0, nado, pad;
+};
+
+enum { F_SDIP = 1, F_CDIP = 2, F_DENSE = 4 };
+
+struct StageParams {
+  const InstDev* insts;
+  const WorkItem* items;
+  const int32_t* rowptr;
+  const int2* ent;
+  const double2* coef;  // 5 per mode: g, cLm, cRm, cLp, cRp
+  const int32_t* mode;
+  const double2* mat;   // H, sdip, qdiag/qmat, cdiag/cmat
+  const int32_t* resident;  // resident instance ids (resident kernel)
+  const double* field;
+  int64_t step;
+  double dt;
+  const void* xin;
+  void* xout;
+  double2* y;
+  double2* ynew;
+  double2* traj;
+  int64_t traj_stride;
+  int64_t save_slot;  // streamed: slot for this step (-1 = none)
+  int64_t nsteps;     // resident kernel: steps to run
+  int64_t save_every; // resident kernel
+  int32_t item_begin, item_end;
+  int32_t xr[9];  // XCD ranges of the item queue (item_begin-relative), xr[8] = count
+  int32_t use_xcd;
+  int32_t col_override;  // >= 0: field column used instead of the stage's own (RHS-only evaluation)
+  int32_t d2max, qmax, Kmax, Lmax;
+};
+
+// LDS carve-up shared by both kernels.
+template <typename T>
+struct Lds {
+  cpx<T>* H;
+  cpx<T>* Q;
+  cpx<T>* C;
+  int32_t* M;
+  T* S;
+  char* rest;
+};
+
+template <typename T>
+__device__ __forceinline__ Lds<T> carve(char* base, const StageParams& p) {
+  Lds<T> l;
+  size_t o = 0;
+  l.H = (cpx<T>*)(base + o);
+  o += sizeof(cpx<T>) * p.d2max;
+  l.Q = (cpx<T>*)(base + o);
+  o += sizeof(cpx<T>) * p.qmax;
+  l.C = (cpx<T>*)(base + o);
+  o += sizeof(cpx<T>) * 5 * p.Kmax;
+  l.M = (int32_t*)(base + o);
+  o += sizeof(int32_t) * p.Kmax;
+  l.S = (T*)(base + o);
+  o += sizeof(T) * (p.Lmax + 2);
+  o = (o + 15) & ~size_t(15);
+  l.rest = base + o;
+  return l;
+}
+
+__host__ __device__ inline size_t lds_header_bytes(int d2max, int qmax, int Kmax, int Lmax, size_t tsize) {
+  size_t o = 2 * tsize * (size_t)d2max + 2 * tsize * (size_t)qmax + 2 * tsize * 5 * (size_t)Kmax + 4 * (size_t)Kmax +
+             tsize * (size_t)(Lmax + 2);
+  return (o + 15) & ~size_t(15);
+}
+
+__device__ __forceinline__ int stage_col(int stage) { return stage == 0 ? 0 : (stage == 3 ? 2 : 1); }
+
+// Load H(t), Q(t) diagonals, per-mode coefficients and sqrt table of one instance into LDS.
+template <typename T>
+__device__ __forceinline__ void load_instance(const StageParams& p, const InstDev& I, const Lds<T>& l, int c,
+                                              int64_t step) {
+  const int tid = threadIdx.x;
+  double fs = 0.0, fc = 0.0;
+  if (p.field) {
+    fs = p.field[6 * step + c];
+    fc = p.field[6 * step + 3 + c];
+  }
+  const double2* H = p.mat + I.h_off;
+  const bool sd = (I.flags & F_SDIP) != 0;
+  for (int e = tid; e < I.d2; e += kThreads) {
+    double2 h = H[e];
+    if (sd) {
+      double2 s = H[I.d2 + e];
+      h.x += s.x * fs;
+      h.y += s.y * fs;
+    }
+    l.H[e] = mk<T>((T)h.x, (T)h.y);
+  }
+  if (!(I.flags & F_DENSE)) {
+    const int nq = I.ncoup * I.d;
+    const double2* q = p.mat + I.q_off;
+    const bool cd = (I.flags & F_CDIP) != 0;
+    for (int e = tid; e < nq; e += kThreads) {
+      double2 v = q[e];
+      if (cd) {
+        double2 s = q[nq + e];
+        v.x += s.x * fc;
+        v.y += s.y * fc;
+      }
+      l.Q[e] = mk<T>((T)v.x, (T)v.y);
+    }
+  }
+  for (int e = tid; e < 5 * I.K; e += kThreads) {
+    double2 v = p.coef[I.coef_off + e];
+    l.C[e] = mk<T>((T)v.x, (T)v.y);
+  }
+  for (int e = tid; e < I.K; e += kThreads) l.M[e] = p.mode[I.mode_off + e];
+  for (int e = tid; e < I.L + 2; e += kThreads) l.S[e] = (T)sqrt((double)e);
+}
+
+template <typename T, typename S>
+__device__ __forceinline__ cpx<T> ldc(const S* base, int64_t idx);
+
+template <>
+__device__ __forceinline__ cpx<double> ldc<double, cpx_store<double>>(const cpx_store<double>* b, int64_t i) {
+  return ld(b + i);
+}
+template <>
+__device__ __forceinline__ cpx<float> ldc<float, cpx_store<float>>(const cpx_store<float>* b, int64_t i) {
+  return ld(b + i);
+}
+template <>
+__device__ __forceinline__ cpx<float> ldc<float, cpx_store<double>>(const cpx_store<double>* b, int64_t i) {
+  return cvt<float>(ld(b + i));
+}
+template <>
+__device__ __forceinline__ cpx<double> ldc<double, cpx<double>>(const cpx<double>* b, int64_t i) {
+  return b[i];
+}
+template <>
+__device__ __forceinline__ cpx<float> ldc<float, cpx<float>>(const cpx<float>* b, int64_t i) {
+  return b[i];
+}
+template <>
+__device__ __forceinline__ cpx<float> ldc<float, cpx<double>>(const cpx<double>* b, int64_t i) {
+  return cvt<float>(b[i]);
+}
+
+// Right-hand side of element (i, j) of ADO `a`.
+//   own : the ADO's own d x d matrix (LDS), in T
+//   xin : base of the instance's stage input (ADO 0), indexable by ldc
+template <typename T, typename OS, typename XS>
+__device__ __forceinline__ cpx<T> element_rhs(const StageParams& p, const InstDev& I, const Lds<T>& l,
+                                              const OS* own, const XS* xin, int a, int i, int j,
+                                              double fc) {
+  const int d = I.d, d2 = I.d2;
+  cpx<T> c = mk<T>(0, 0);
+  for (int k = 0; k < d; ++k) {
+    cmac(c, l.H[i * d + k], ldc<T>(own, k * d + j));
+    cmsub(c, ldc<T>(own, i * d + k), l.H[k * d + j]);
+  }
+  cpx<T> r = mul_mi(c);
+  cpx<T> damp = mk<T>(0, 0);
+  const int r0 = p.rowptr[I.rowptr_off + a], r1 = p.rowptr[I.rowptr_off + a + 1];
+  const int2* ent = p.ent + I.ent_off;
+  const bool dense = (I.flags & F_DENSE) != 0;
+  for (int t = r0; t < r1; ++t) {
+    const int2 en = ent[t];
+    const int k = en.y & 0xffff;
+    const int nf = (en.y >> 16) & 0x7f;
+    const bool plus = en.y < 0;
+    const cpx<T>* C = l.C + 5 * k;
+    if (!plus) damp = damp + C[0] * (T)nf;
+    const cpx<T> cL = plus ? C[3] : C[1];
+    const cpx<T> cR = plus ? C[4] : C[2];
+    const T sq = l.S[nf];
+    const int m = l.M[k];
+    const int64_t nb = (int64_t)en.x * d2;
+    if (!dense) {
+      cpx<T> coef = (cL * l.Q[m * d + i] + cR * l.Q[m * d + j]) * sq;
+      if (coef.x != (T)0 || coef.y != (T)0) cmac(r, coef, ldc<T>(xin, nb + i * d + j));
+    } else {
+      // dense coupling: cL (Q X)_{ij} + cR (X Q)_{ij}, Q read from the FP64 matrix pool
+      const double2* Qm = p.mat + I.q_off + (int64_t)m * d2;
+      const double2* Cm = (I.flags & F_CDIP) ? Qm + (int64_t)I.ncoup * d2 : nullptr;
+      cpx<T> s1 = mk<T>(0, 0), s2 = mk<T>(0, 0);
+      for (int q = 0; q < d; ++q) {
+        double2 qa = Qm[i * d + q], qb = Qm[q * d + j];
+        if (Cm) {
+          double2 ca = Cm[i * d + q], cb = Cm[q * d + j];
+          qa.x += ca.x * fc;
+          qa.y += ca.y * fc;
+          qb.x += cb.x * fc;
+          qb.y += cb.y * fc;
+        }
+        cmac(s1, mk<T>((T)qa.x, (T)qa.y), ldc<T>(xin, nb + q * d + j));
+        cmac(s2, ldc<T>(xin, nb + i * d + q), mk<T>((T)qb.x, (T)qb.y));
+      }
+      cpx<T> v = cL * s1 + cR * s2;
+      r = r + v * sq;
+    }
+  }
+  cmsub(r, damp, ldc<T>(own, i * d + j));
+  return r;
+}
+
+// ------------------------------------------------------------------ streamed stage kernel
+// STAGE 0..3: RK4 stage; STAGE 4: out = RHS(xin) only (FP64 operator check).
+template <typename T, typename TI, typename TO, int STAGE>
+__global__ __launch_bounds__(kThreads) void heom_stage_kernel(StageParams p) {
+  extern __shared__ __align__(16) char smem[];
+  const Lds<T> l = carve<T>(smem, p);
+  cpx<T>* sX = (cpx<T>*)l.rest;
+  const int tid = threadIdx.x;
+  const cpx_store<TI>* xin = (const cpx_store<TI>*)p.xin;
+  cpx_store<TO>* xout = (cpx_store<TO>*)p.xout;
+  const double dt = p.dt;
+  const int col = p.col_override >= 0 ? p.col_override : stage_col(STAGE & 3);
+  double fc = 0.0;
+  if (p.field) fc = p.field[6 * p.step + 3 + col];
+
+  int it0, itstep, itend;
+  if (p.use_xcd) {
+    const int x = blockIdx.x & 7, j = blockIdx.x >> 3, nbx = gridDim.x >> 3;
+    it0 = p.item_begin + p.xr[x] + j;
+    itend = p.item_begin + p.xr[x + 1];
+    itstep = nbx;
+  } else {
+    it0 = p.item_begin + blockIdx.x;
+    itend = p.item_end;
+    itstep = gridDim.x;
+  }
+  for (int it = it0; it < itend; it += itstep) {
+    const WorkItem w = p.items[it];
+    const InstDev I = p.insts[w.inst];
+    __syncthreads();
+    load_instance<T>(p, I, l, col, p.step);
+    const int d2 = I.d2, d = I.d, G = I.grp;
+    const int nslot = kThreads / G;
+    const int slot = tid / G, le = tid % G;
+    cpx<T>* own = sX + slot * d2;
+    const cpx_store<TI>* xi = xin + I.off;
+    for (int a0 = w.ado0; a0 < w.ado0 + w.nado; a0 += nslot) {
+      const int a = a0 + slot;
+      const bool act = a < w.ado0 + w.nado;
+      __syncthreads();
+      if (act)
+        for (int e = le; e < d2; e += G) own[e] = ldc<T>(xi, (int64_t)a * d2 + e);
+      __syncthreads();
+      if (act) {
+        for (int e = le; e < d2; e += G) {
+          const int i = e / d, j = e - (e / d) * d;
+          const cpx<T> r = element_rhs<T>(p, I, l, own, xi, a, i, j, fc);
+          const int64_t g = I.off + (int64_t)a * d2 + e;
+          const double rx = (double)r.x, ry = (double)r.y;
+          if (STAGE == 4) {
+            ((double2*)p.xout)[g] = make_double2(rx, ry);
+          } else if (STAGE == 0) {
+            const double2 yv = p.y[g];
+            p.ynew[g] = make_double2(yv.x + rx * (dt / 6), yv.y + ry * (dt / 6));
+            st(xout + g, mk<TO>((TO)(yv.x + rx * (dt / 2)), (TO)(yv.y + ry * (dt / 2))));
+          } else if (STAGE == 1 || STAGE == 2) {
+            const double2 yv = p.y[g];
+            const double2 an = p.ynew[g];
+            p.ynew[g] = make_double2(an.x + rx * (dt / 3), an.y + ry * (dt / 3));
+            const double h = STAGE == 1 ? dt / 2 : dt;
+            st(xout + g, mk<TO>((TO)(yv.x + rx * h), (TO)(yv.y + ry * h)));
+          } else {
+            const double2 an = p.ynew[g];
+            const double2 yn = make_double2(an.x + rx * (dt / 6), an.y + ry * (dt / 6));
+            p.y[g] = yn;
+            if (a == 0 && p.traj && p.save_slot >= 0) p.traj[p.save_slot * p.traj_stride + I.traj_off + e] = yn;
+          }
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ LDS-resident kernel
+// One workgroup owns one small instance for the whole run: y, ynew and both
+// stage buffers live in LDS, so HBM is touched only to load the initial state,
+// write trajectory samples and store the final state.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void heom_resident_kernel(StageParams p) {
+  extern __shared__ __align__(16) char smem[];
+  const Lds<T> l = carve<T>(smem, p);
+  const InstDev I = p.insts[p.resident[blockIdx.x]];
+  const int n = I.nmax * I.d2, d2 = I.d2, d = I.d;
+  cpx<double>* Y = (cpx<double>*)l.rest;
+  cpx<double>* Yn = Y + n;
+  cpx<T>* X0 = (cpx<T>*)(Yn + n);
+  cpx<T>* X1 = X0 + n;
+  const int tid = threadIdx.x;
+  const double dt = p.dt;
+  for (int e = tid; e < n; e += kThreads) {
+    double2 v = p.y[I.off + e];
+    Y[e] = mk(v.x, v.y);
+  }
+  for (int64_t s = 0; s < p.nsteps; ++s) {
+    const int64_t step = p.step + s;
+#pragma unroll 1
+    for (int stage = 0; stage < 4; ++stage) {
+      __syncthreads();
+      load_instance<T>(p, I, l, stage_col(stage), step);
+      double fc = 0.0;
+      if (p.field) fc = p.field[6 * step + 3 + stage_col(stage)];
+      __syncthreads();
+      const cpx<T>* xs = stage == 1 ? X0 : (stage == 2 ? X1 : X0);
+      cpx<T>* xd = stage == 1 ? X1 : X0;
+      for (int e = tid; e < n; e += kThreads) {
+        const int a = e / d2, r = e - a * d2;
+        const int i = r / d, j = r - (r / d) * d;
+        cpx<T> rr;
+        if (stage == 0) {
+          // RHS input is y (double), converted to T as it is read
+          rr = element_rhs<T>(p, I, l, Y + a * d2, Y, a, i, j, fc);
+        } else {
+          rr = element_rhs<T>(p, I, l, xs + a * d2, xs, a, i, j, fc);
+        }
+        const double rx = (double)rr.x, ry = (double)rr.y;
+        const cpx<double> yv = Y[e];
+        if (stage == 0) {
+          Yn[e] = mk(yv.x + rx * (dt / 6), yv.y + ry * (dt / 6));
+          xd[e] = mk<T>((T)(yv.x + rx * (dt / 2)), (T)(yv.y + ry * (dt / 2)));
+        } else if (stage < 3) {
+          const cpx<double> an = Yn[e];
+          Yn[e] = mk(an.x + rx * (dt / 3), an.y + ry * (dt / 3));
+          const double h = stage == 1 ? dt / 2 : dt;
+          xd[e] = mk<T>((T)(yv.x + rx * h), (T)(yv.y + ry * h));
+        }
+        if (stage == 3) {
+          const cpx<double> an = Yn[e];
+          // Y is not read by other threads in stage 3 (its input is X0), so it can be updated in place
+          Y[e] = mk(an.x + rx * (dt / 6), an.y + ry * (dt / 6));
+        }
+      }
+    }
+    __syncthreads();
+    if (p.traj && ((s + 1) % p.save_every) == 0) {
+      const int64_t slot = (s + 1) / p.save_every - 1;
+      for (int e = tid; e < d2; e += kThreads) {
+        p.traj[slot * p.traj_stride + I.traj_off + e] = make_double2(Y[e].x, Y[e].y);
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < n; e += kThreads) p.y[I.off + e] = make_double2(Y[e].x, Y[e].y);
+}
+
+}  // namespace
+
+// ====================================================================== host side
+namespace {
+
+// Binomial table C(n, r) for n <= nmaxn, r <= rmax, saturating at 2^62.
+struct Binom {
+  int rows = 0, cols = 0;
+  std::vector<int64_t> v;
+  Binom(int n, int r) : rows(n + 1), cols(r + 1), v((size_t)(n + 1) * (r + 1), 0) {
+    const int64_t cap = int64_t(1) << 62;
+    for (int i = 0; i <= n; ++i) {
+      at(i, 0) = 1;
+      for (int j = 1; j <= r && j <= i; ++j) {
+        int64_t s = at(i - 1, j - 1) + (j <= i - 1 ? at(i - 1, j) : 0);
+        at(i, j) = s > cap ? cap : s;
+      }
+    }
+  }
+  int64_t& at(int n, int r) { return v[(size_t)n * cols + r]; }
+  int64_t get(int n, int r) const { return (n < 0 || r < 0 || r > n || r >= cols) ? 0 : v[(size_t)n * cols + r]; }
+};
+
+// Tables of one (K, L) hierarchy.  Ranks follow the combinatorial number
+// system the reference uses (deom.py:555-565): rank(n) = sum_i C(s_i + i, i+1)
+// with s_i = n_0 + ... + n_i.  Here every rank is unranked directly (greedy
+// decomposition), not generated by the reference's tier-by-tier search, and
+// neighbour ranks use rank(n + e_k) - rank(n) = sum_{i>=k} C(s_i + i, i).
+struct Tables {
+  int K = 0, L = 0, nmax = 0;
+  std::vector<int64_t> keys;    // [nmax][K]
+  std::vector<int32_t> rowptr;  // [nmax+1]
+  std::vector<int2> ent;        // reference order: for k: minus (n_k>0), plus (|n|<L)
+};
+
+int build_tables(int K, int L, Tables& t) {
+  Binom C(K + L + 1, K + 1);
+  const int64_t nm = C.get(K + L, K);
+  if (nm <= 0 || nm > (int64_t)INT32_MAX) return fail(PYQED_EINVAL, "hierarchy too large: C(K+L, L) exceeds 2^31-1");
+  t.K = K;
+  t.L = L;
+  t.nmax = (int)nm;
+  t.keys.assign((size_t)t.nmax * K, 0);
+  t.rowptr.assign((size_t)t.nmax + 1, 0);
+  t.ent.clear();
+  std::vector<int64_t> s(K), cvec(K + 1);
+  for (int r = 0; r < t.nmax; ++r) {
+    // unrank: N = sum_{j=1..K} C(c_j, j), c_1 < c_2 < ... < c_K
+    int64_t N = r;
+    for (int j = K; j >= 1; --j) {
+      int c = j - 1;
+      while (C.get(c + 1, j) <= N) ++c;
+      N -= C.get(c, j);
+      cvec[j] = c;
+    }
+    int tier = 0;
+    for (int i = 0; i < K; ++i) {
+      s[i] = cvec[i + 1] - i;  // s_i = c_{i+1} - i
+      const int64_t n = i == 0 ? s[0] : s[i] - s[i - 1];
+      t.keys[(size_t)r * K + i] = n;
+    }
+    tier = K ? (int)s[K - 1] : 0;
+    // suffix sums for neighbour ranks
+    std::vector<int64_t> up(K + 1, 0), dn(K + 1, 0);
+    for (int i = K - 1; i >= 0; --i) {
+      up[i] = up[i + 1] + C.get((int)s[i] + i, i);
+      dn[i] = dn[i + 1] + C.get((int)s[i] + i - 1, i);
+    }
+    for (int k = 0; k < K; ++k) {
+      const int64_t nk = t.keys[(size_t)r * K + k];
+      if (nk > 0) {
+        int2 e;
+        e.x = (int)(r - dn[k]);
+        e.y = (k & 0xffff) | ((int)(nk & 0x7f) << 16);
+        t.ent.push_back(e);
+      }
+      if (tier < L) {
+        int2 e;
+        e.x = (int)(r + up[k]);
+        e.y = (int)((uint32_t)(k & 0xffff) | ((uint32_t)((nk + 1) & 0x7f) << 16) | 0x80000000u);
+        t.ent.push_back(e);
+      }
+    }
+    t.rowptr[r + 1] = (int32_t)t.ent.size();
+    if (t.ent.size() > (size_t)INT32_MAX / 2) return fail(PYQED_EINVAL, "neighbour table too large");
+  }
+  return PYQED_OK;
+}
+
+template <typename K>
+void* kernel_ptr(K k) {
+  return reinterpret_cast<void*>(k);
+}
+
+}  // namespace
+
+struct pyqed_heom_plan {
+  int n_inst = 0, flags = 0, device = 0;
+  std::vector<InstDev> inst;
+  std::vector<int64_t> nmax;
+  std::map<std::pair<int, int>, Tables> tabs;
+  std::vector<int> inst_K, inst_L;
+  int64_t n_elems = 0, n_ados = 0, traj_stride = 0;
+  int d2max = 1, qmax = 1, Kmax = 1, Lmax = 0;
+  std::vector<WorkItem> items;  // streamed items, then items of resident instances
+  int n_items_stream = 0;
+  int xr[9] = {0};
+  std::vector<int32_t> resident;
+  int64_t res_elems = 0, stream_elems = 0;
+  int num_cu = 256, max_lds = 65536;
+  // device copies
+  InstDev* d_inst = nullptr;
+  WorkItem* d_items = nullptr;
+  int32_t* d_rowptr = nullptr;
+  int2* d_ent = nullptr;
+  double2* d_coef = nullptr;
+  int32_t* d_mode = nullptr;
+  double2* d_mat = nullptr;
+  int32_t* d_res = nullptr;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+};
+
+namespace {
+
+template <typename V>
+int upload(const std::vector<V>& h, V** d) {
+  if (h.empty()) {
+    *d = nullptr;
+    return PYQED_OK;
+  }
+  PQA_HIP(hipMalloc((void**)d, sizeof(V) * h.size()));
+  PQA_HIP(hipMemcpy(*d, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice));
+  return PYQED_OK;
+}
+
+size_t stream_lds_bytes(const pyqed_heom_plan* P, size_t tsize) {
+  size_t h = lds_header_bytes(P->d2max, P->qmax, P->Kmax, P->Lmax, tsize);
+  size_t x = 2 * tsize * (size_t)std::max(256, 4 * P->d2max);
+  return h + x;
+}
+
+size_t resident_lds_bytes(const pyqed_heom_plan* P, int64_t n_el, size_t tsize) {
+  size_t h = lds_header_bytes(P->d2max, P->qmax, P->Kmax, P->Lmax, tsize);
+  return h + (size_t)n_el * (32 + 2 * 2 * tsize);
+}
+
+int group_size(int d2) {
+  int g = 4;
+  while (g < d2 && g < 64) g <<= 1;
+  return g;
+}
+
+void destroy_plan(pyqed_heom_plan* P) {
+  if (!P) return;
+  (void)hipFree(P->d_inst);
+  (void)hipFree(P->d_items);
+  (void)hipFree(P->d_rowptr);
+  (void)hipFree(P->d_ent);
+  (void)hipFree(P->d_coef);
+  (void)hipFree(P->d_mode);
+  (void)hipFree(P->d_mat);
+  (void)hipFree(P->d_res);
+  if (P->side) (void)hipStreamDestroy(P->side);
+  if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
+  if (P->ev_join) (void)hipEventDestroy(P->ev_join);
+  delete P;
+}
+
+std::complex<double> cz(const double* p, int i) { return std::complex<double>(p[2 * i], p[2 * i + 1]); }
+
+}  // namespace
+
+extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_inst, int32_t flags,
+                                      pyqed_heom_plan** out) {
+  if (!in || !out || n_inst <= 0) return fail(PYQED_EINVAL, "plan_create: null argument or n_inst <= 0");
+  *out = nullptr;
+  pyqed_heom_plan* P = new pyqed_heom_plan();
+  P->n_inst = n_inst;
+  P->flags = flags;
+  const bool dense = (flags & PYQED_PLAN_DENSE_Q) != 0;
+  // ---- validate and build tables
+  for (int i = 0; i < n_inst; ++i) {
+    const pyqed_heom_instance& s = in[i];
+    if (s.d < 1 || s.d > kMaxD) {
+      destroy_plan(P);
+      return fail(PYQED_EINVAL, "instance " + std::to_string(i) + ": d must be in 1..32");
+    }
+    if (s.K < 0 || s.K > kMaxK || s.L < 0 || s.L > kMaxL || s.ncoup < 1) {
+      destroy_plan(P);
+      return fail(PYQED_EINVAL, "instance " + std::to_string(i) + ": K in 0..255, L in 0..126, ncoup >= 1 required");
+    }
+    if (!s.H || (s.K > 0 && (!s.mode || !s.expn || !s.etal || !s.etar || !s.etaa)) || (!dense && !s.qdiag) ||
+        (dense && !s.qmat)) {
+      destroy_plan(P);
+      return fail(PYQED_EINVAL, "instance " + std::to_string(i) + ": missing H, bath or coupling arrays");
+    }
+    for (int k = 0; k < s.K; ++k) {
+      if (s.mode[k] < 0 || s.mode[k] >= s.ncoup) {
+        destroy_plan(P);
+        return fail(PYQED_EINVAL, "instance " + std::to_string(i) + ": mode[" + std::to_string(k) +
+                                      "] out of range of the coupling operators");
+      }
+      if (std::abs(cz(s.etaa, k)) == 0.0) {
+        destroy_plan(P);
+        return fail(PYQED_EINVAL, "instance " + std::to_string(i) + ": etaa[" + std::to_string(k) +
+                                      "] == 0 (the operator divides by sqrt(etaa))");
+      }
+    }
+    auto key = std::make_pair((int)s.K, (int)s.L);
+    if (!P->tabs.count(key)) {
+      Tables t;
+      int rc = build_tables(s.K, s.L, t);
+      if (rc) {
+        destroy_plan(P);
+        return rc;
+      }
+      P->tabs.emplace(key, std::move(t));
+    }
+  }
+  // ---- pack pools
+  std::vector<int32_t> rowptr;
+  std::vector<int2> ent;
+  std::map<std::pair<int, int>, std::pair<int, int>> tab_off;
+  for (auto& kv : P->tabs) {
+    tab_off[kv.first] = {(int)rowptr.size(), (int)ent.size()};
+    rowptr.insert(rowptr.end(), kv.second.rowptr.begin(), kv.second.rowptr.end());
+    ent.insert(ent.end(), kv.second.ent.begin(), kv.second.ent.end());
+  }
+  std::vector<double2> coef, mat;
+  std::vector<int32_t> mode;
+  int64_t off = 0, toff = 0, ados = 0;
+  for (int i = 0; i < n_inst; ++i) {
+    const pyqed_heom_instance& s = in[i];
+    const Tables& t = P->tabs.at({s.K, s.L});
+    InstDev I;
+    std::memset(&I, 0, sizeof(I));
+    I.d = s.d;
+    I.d2 = s.d * s.d;
+    I.K = s.K;
+    I.L = s.L;
+    I.ncoup = s.ncoup;
+    I.nmax = t.nmax;
+    I.flags = (s.sdip ? F_SDIP : 0) | (dense ? (s.cmat ? F_CDIP : 0) : (s.cdiag ? F_CDIP : 0)) | (dense ? F_DENSE : 0);
+    I.grp = group_size(I.d2);
+    I.off = off;
+    I.traj_off = toff;
+    I.rowptr_off = tab_off[{s.K, s.L}].first;
+    I.ent_off = tab_off[{s.K, s.L}].second;
+    I.coef_off = (int)coef.size();
+    I.mode_off = (int)mode.size();
+    for (int k = 0; k < s.K; ++k) {
+      const std::complex<double> I1(0.0, 1.0);
+      const std::complex<double> sq = std::sqrt(cz(s.etaa, k));
+      const std::complex<double> v[5] = {cz(s.expn, k), -I1 * cz(s.etal, k) / sq, I1 * cz(s.etar, k) / sq, -I1 * sq,
+                                         I1 * sq};
+      for (int q = 0; q < 5; ++q) coef.push_back(make_double2(v[q].real(), v[q].imag()));
+      mode.push_back(s.mode[k]);
+    }
+    I.h_off = (int)mat.size();
+    for (int e = 0; e < I.d2; ++e) mat.push_back(make_double2(s.H[2 * e], s.H[2 * e + 1]));
+    if (s.sdip)
+      for (int e = 0; e < I.d2; ++e) mat.push_back(make_double2(s.sdip[2 * e], s.sdip[2 * e + 1]));
+    I.q_off = (int)mat.size();
+    const int nq = dense ? s.ncoup * I.d2 : s.ncoup * s.d;
+    const double* qsrc = dense ? s.qmat : s.qdiag;
+    const double* csrc = dense ? s.cmat : s.cdiag;
+    for (int e = 0; e < nq; ++e) mat.push_back(make_double2(qsrc[2 * e], qsrc[2 * e + 1]));
+    if (csrc)
+      for (int e = 0; e < nq; ++e) mat.push_back(make_double2(csrc[2 * e], csrc[2 * e + 1]));
+    if (mat.size() > (size_t)INT32_MAX || coef.size() > (size_t)INT32_MAX) {
+      destroy_plan(P);
+      return fail(PYQED_EINVAL, "coefficient pools exceed 2^31 entries");
+    }
+    P->inst.push_back(I);
+    P->nmax.push_back(t.nmax);
+    P->inst_K.push_back(s.K);
+    P->inst_L.push_back(s.L);
+    off += (int64_t)t.nmax * I.d2;
+    toff += I.d2;
+    ados += t.nmax;
+    P->d2max = std::max(P->d2max, I.d2);
+    P->qmax = std::max(P->qmax, dense ? 1 : s.ncoup * s.d);
+    P->Kmax = std::max(P->Kmax, s.K);
+    P->Lmax = std::max(P->Lmax, s.L);
+  }
+  P->n_elems = off;
+  P->n_ados = ados;
+  P->traj_stride = toff;
+  // ---- device properties
+  if (hipGetDevice(&P->device) != hipSuccess) {
+    destroy_plan(P);
+    return fail(PYQED_EHIP, "no HIP device");
+  }
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, P->device) == hipSuccess && v > 0)
+    P->num_cu = v;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, P->device) == hipSuccess && v > 0)
+    P->max_lds = v;
+  // ---- residency: the whole instance (FP64 sizes) must fit one workgroup's LDS
+  std::vector<char> is_res(n_inst, 0);
+  if (!(flags & PYQED_PLAN_NO_RESIDENT)) {
+    for (int i = 0; i < n_inst; ++i) {
+      const int64_t n = (int64_t)P->inst[i].nmax * P->inst[i].d2;
+      if (resident_lds_bytes(P, n, sizeof(double)) <= (size_t)P->max_lds) is_res[i] = 1;
+    }
+  }
+  // ---- work items: tiles of ADOs, streamed instances first
+  for (int i = 0; i < n_inst; ++i) (is_res[i] ? P->res_elems : P->stream_elems) += (int64_t)P->inst[i].nmax * P->inst[i].d2;
+  const int64_t target =
+      std::min<int64_t>(8192, std::max<int64_t>(256, P->stream_elems / std::max<int64_t>(1, (int64_t)P->num_cu * 8)));
+  auto tile = [&](int i) {
+    const InstDev& I = P->inst[i];
+    const int nslot = kThreads / I.grp;
+    int64_t per = (target + I.d2 - 1) / I.d2;
+    per = ((per + nslot - 1) / nslot) * nslot;
+    per = std::max<int64_t>(per, nslot);
+    for (int a = 0; a < I.nmax; a += (int)per) {
+      WorkItem w;
+      w.inst = i;
+      w.ado0 = a;
+      w.nado = (int)std::min<int64_t>(per, I.nmax - a);
+      w.pad = 0;
+      P->items.push_back(w);
+    }
+  };
+  for (int i = 0; i < n_inst; ++i)
+    if (!is_res[i]) tile(i);
+  P->n_items_stream = (int)P->items.size();
+  for (int i = 0; i < n_inst; ++i)
+    if (is_res[i]) {
+      tile(i);
+      P->resident.push_back(i);
+    }
+  // XCD ranges over the streamed items, balanced by element count
+  {
+    std::vector<int64_t> pre(P->n_items_stream + 1, 0);
+    for (int k = 0; k < P->n_items_stream; ++k)
+      pre[k + 1] = pre[k] + (int64_t)P->items[k].nado * P->inst[P->items[k].inst].d2;
+    P->xr[0] = 0;
+    for (int x = 1; x < 8; ++x) {
+      const int64_t goal = pre.back() * x / 8;
+      P->xr[x] = (int)(std::lower_bound(pre.begin(), pre.end(), goal) - pre.begin());
+      P->xr[x] = std::max(P->xr[x], P->xr[x - 1]);
+    }
+    P->xr[8] = P->n_items_stream;
+  }
+  if (stream_lds_bytes(P, sizeof(double)) > (size_t)P->max_lds) {
+    destroy_plan(P);
+    return fail(PYQED_EUNSUPPORTED, "instance too large for the streamed kernel's LDS budget");
+  }
+  // ---- upload
+  int rc = 0;
+  if ((rc = upload(P->inst, &P->d_inst)) || (rc = upload(P->items, &P->d_items)) ||
+      (rc = upload(rowptr, &P->d_rowptr)) || (rc = upload(ent, &P->d_ent)) || (rc = upload(coef, &P->d_coef)) ||
+      (rc = upload(mode, &P->d_mode)) || (rc = upload(mat, &P->d_mat)) || (rc = upload(P->resident, &P->d_res))) {
+    destroy_plan(P);
+    return rc;
+  }
+  if (hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming) != hipSuccess) {
+    destroy_plan(P);
+    return fail(PYQED_EHIP, "stream/event creation failed");
+  }
+  *out = P;
+  return PYQED_OK;
+}
+
+extern "C" int pyqed_heom_plan_destroy(pyqed_heom_plan* plan) {
+  destroy_plan(plan);
+  return PYQED_OK;
+}
+
+extern "C" int pyqed_heom_plan_query(const pyqed_heom_plan* P, int64_t* n_elems, int64_t* n_ados,
+                                     int64_t* ws64, int64_t* wsmix) {
+  if (!P) return fail(PYQED_EINVAL, "plan_query: null plan");
+  if (n_elems) *n_elems = P->n_elems;
+  if (n_ados) *n_ados = P->n_ados;
+  if (ws64) *ws64 = P->n_elems * 48;
+  if (wsmix) *wsmix = P->n_elems * 32;
+  return PYQED_OK;
+}
+
+extern "C" int pyqed_heom_plan_layout(const pyqed_heom_plan* P, int64_t* so, int64_t* nm, int64_t* to) {
+  if (!P) return fail(PYQED_EINVAL, "plan_layout: null plan");
+  for (int i = 0; i < P->n_inst; ++i) {
+    if (so) so[i] = P->inst[i].off;
+    if (nm) nm[i] = P->inst[i].nmax;
+    if (to) to[i] = P->inst[i].traj_off;
+  }
+  return PYQED_OK;
+}
+
+extern "C" int pyqed_heom_plan_stats(const pyqed_heom_plan* P, int64_t* s) {
+  if (!P || !s) return fail(PYQED_EINVAL, "plan_stats: null argument");
+  s[0] = P->n_items_stream;
+  s[1] = (int64_t)P->resident.size();
+  s[2] = P->stream_elems;
+  s[3] = P->res_elems;
+  return PYQED_OK;
+}
+
+namespace {
+
+void export_tables(const Tables& t, int64_t* keys, int32_t* plus, int32_t* minus) {
+  const int K = t.K;
+  if (keys) std::memcpy(keys, t.keys.data(), sizeof(int64_t) * t.keys.size());
+  if (plus) std::fill(plus, plus + (size_t)t.nmax * K, -1);
+  if (minus) std::fill(minus, minus + (size_t)t.nmax * K, -1);
+  for (int a = 0; a < t.nmax; ++a)
+    for (int q = t.rowptr[a]; q < t.rowptr[a + 1]; ++q) {
+      const int2 e = t.ent[q];
+      const int k = e.y & 0xffff;
+      if (e.y < 0) {
+        if (plus) plus[(size_t)a * K + k] = e.x;
+      } else if (minus) {
+        minus[(size_t)a * K + k] = e.x;
+      }
+    }
+}
+
+}  // namespace
+
+extern "C" int pyqed_heom_tables_export(const pyqed_heom_plan* P, int32_t inst, int64_t* keys, int32_t* plus,
+                                        int32_t* minus) {
+  if (!P || inst < 0 || inst >= P->n_inst) return fail(PYQED_EINVAL, "tables_export: bad plan or instance");
+  export_tables(P->tabs.at({P->inst_K[inst], P->inst_L[inst]}), keys, plus, minus);
+  return PYQED_OK;
+}
+
+
+extern "C" int pyqed_heom_tables_size(int32_t K, int32_t L, int64_t* nmax) {
+  if (!nmax || K < 0 || K > kMaxK || L < 0 || L > kMaxL) return fail(PYQED_EINVAL, "tables_size: bad K or L");
+  Binom C(K + L + 1, K + 1);
+  const int64_t n = C.get(K + L, K);
+  if (n <= 0 || n > (int64_t)INT32_MAX) return fail(PYQED_EINVAL, "hierarchy too large: C(K+L, L) exceeds 2^31-1");
+  *nmax = n;
+  return PYQED_OK;
+}
+
+extern "C" int pyqed_heom_tables_build(int32_t K, int32_t L, int64_t* keys, int32_t* plus, int32_t* minus) {
+  if (K < 0 || K > kMaxK || L < 0 || L > kMaxL) return fail(PYQED_EINVAL, "tables_build: bad K or L");
+  Tables t;
+  int rc = build_tables(K, L, t);
+  if (rc) return rc;
+  export_tables(t, keys, plus, minus);
+  return PYQED_OK;
+}
+
+namespace {
+
+StageParams base_params(const pyqed_heom_plan* P) {
+  StageParams p;
+  std::memset(&p, 0, sizeof(p));
+  p.insts = P->d_inst;
+  p.items = P->d_items;
+  p.rowptr = P->d_rowptr;
+  p.ent = P->d_ent;
+  p.coef = P->d_coef;
+  p.mode = P->d_mode;
+  p.mat = P->d_mat;
+  p.resident = P->d_res;
+  p.traj_stride = P->traj_stride;
+  p.d2max = P->d2max;
+  p.qmax = P->qmax;
+  p.Kmax = P->Kmax;
+  p.Lmax = P->Lmax;
+  p.save_slot = -1;
+  p.col_override = -1;
+  return p;
+}
+
+template <typename Kern>
+int launch(Kern k, int grid, size_t lds, hipStream_t s, const StageParams& p) {
+  if (lds > 65536) PQA_HIP(hipFuncSetAttribute(kernel_ptr(k), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), lds, s, p);
+  PQA_HIP(hipGetLastError());
+  return PYQED_OK;
+}
+
+int stream_grid(const pyqed_heom_plan* P, int n_items) {
+  int g = std::min(n_items, P->num_cu * 8);
+  g = ((g + 7) / 8) * 8;
+  return std::max(g, 8);
+}
+
+template <typename T>
+int run_typed(pyqed_heom_plan* P, double2* y, char* ws, const double* field, int64_t step0, int64_t nsteps,
+              double dt, double2* traj, int64_t save_every, hipStream_t st) {
+  const int64_t n = P->n_elems;
+  double2* ynew = (double2*)ws;
+  cpx_store<T>* x0 = (cpx_store<T>*)(ws + 16 * n);
+  cpx_store<T>* x1 = x0 + n;
+  StageParams p = base_params(P);
+  p.dt = dt;
+  p.y = y;
+  p.ynew = ynew;
+  p.traj = traj;
+  p.field = field;
+  if (!P->resident.empty()) {
+    PQA_HIP(hipEventRecord(P->ev_fork, st));
+    PQA_HIP(hipStreamWaitEvent(P->side, P->ev_fork, 0));
+    StageParams r = p;
+    r.step = step0;
+    r.nsteps = nsteps;
+    r.save_every = save_every;
+    int64_t nres_max = 0;
+    for (int i : P->resident) nres_max = std::max<int64_t>(nres_max, (int64_t)P->inst[i].nmax * P->inst[i].d2);
+    const size_t lds = resident_lds_bytes(P, nres_max, sizeof(T));
+    int rc = launch(heom_resident_kernel<T>, (int)P->resident.size(), lds, P->side, r);
+    if (rc) return rc;
+    PQA_HIP(hipEventRecord(P->ev_join, P->side));
+  }
+  if (P->n_items_stream > 0) {
+    p.item_begin = 0;
+    p.item_end = P->n_items_stream;
+    p.use_xcd = 1;
+    std::memcpy(p.xr, P->xr, sizeof(p.xr));
+    const int grid = stream_grid(P, P->n_items_stream);
+    const size_t lds = stream_lds_bytes(P, sizeof(T));
+    for (int64_t s = 0; s < nsteps; ++s) {
+      p.step = step0 + s;
+      int rc;
+      p.xin = y;
+      p.xout = x0;
+      if ((rc = launch(heom_stage_kernel<T, double, T, 0>, grid, lds, st, p))) return rc;
+      p.xin = x0;
+      p.xout = x1;
+      if ((rc = launch(heom_stage_kernel<T, T, T, 1>, grid, lds, st, p))) return rc;
+      p.xin = x1;
+      p.xout = x0;
+      if ((rc = launch(heom_stage_kernel<T, T, T, 2>, grid, lds, st, p))) return rc;
+      p.xin = x0;
+      p.xout = nullptr;
+      p.save_slot = (traj && ((s + 1) % save_every) == 0) ? (s + 1) / save_every - 1 : -1;
+      if ((rc = launch(heom_stage_kernel<T, T, T, 3>, grid, lds, st, p))) return rc;
+      p.save_slot = -1;
+    }
+  }
+  if (!P->resident.empty()) PQA_HIP(hipStreamWaitEvent(st, P->ev_join, 0));
+  return PYQED_OK;
+}
+
+}  // namespace
+
+extern "C" int pyqed_heom_run(pyqed_heom_plan* P, int32_t precision, void* y, void* workspace, int64_t ws_bytes,
+                              const double* field, int64_t step0, int64_t nsteps, double dt, void* traj,
+                              int64_t save_every, void* stream) {
+  if (!P || !y || !workspace) return fail(PYQED_EINVAL, "heom_run: null plan, state or workspace");
+  if (nsteps < 0 || step0 < 0) return fail(PYQED_EINVAL, "heom_run: negative step count");
+  if (save_every <= 0) save_every = 1;
+  const int64_t need = precision == PYQED_PREC_FP64 ? P->n_elems * 48 : P->n_elems * 32;
+  if (precision != PYQED_PREC_FP64 && precision != PYQED_PREC_MIXED)
+    return fail(PYQED_EINVAL, "heom_run: precision must be PYQED_PREC_FP64 or PYQED_PREC_MIXED");
+  if (ws_bytes < need)
+    return fail(PYQED_EINVAL, "heom_run: workspace too small (" + std::to_string(ws_bytes) + " < " +
+                                  std::to_string(need) + " bytes)");
+  if (nsteps == 0) return PYQED_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (precision == PYQED_PREC_FP64)
+    return run_typed<double>(P, (double2*)y, (char*)workspace, field, step0, nsteps, dt, (double2*)traj, save_every,
+                             st);
+  return run_typed<float>(P, (double2*)y, (char*)workspace, field, step0, nsteps, dt, (double2*)traj, save_every, st);
+}
+
+extern "C" int pyqed_heom_rhs(pyqed_heom_plan* P, const void* x, void* out, const double* field, int64_t step,
+                              int32_t col, void* stream) {
+  if (!P || !x || !out) return fail(PYQED_EINVAL, "heom_rhs: null argument");
+  if (col < 0 || col > 2) return fail(PYQED_EINVAL, "heom_rhs: col must be 0, 1 or 2");
+  StageParams p = base_params(P);
+  p.field = field;
+  p.step = step;
+  p.xin = x;
+  p.xout = out;
+  p.item_begin = 0;
+  p.item_end = (int)P->items.size();
+  p.use_xcd = 0;
+  p.col_override = col;
+  const int grid = std::max(1, std::min((int)P->items.size(), P->num_cu * 8));
+  const size_t lds = stream_lds_bytes(P, sizeof(double));
+  return launch(heom_stage_kernel<double, double, double, 4>, grid, lds, (hipStream_t)stream, p);
+}

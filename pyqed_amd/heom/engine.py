@@ -1,0 +1,255 @@
+"""GPU engine behind DEOMSolver and HEOMBatch: one libpyqed_amd plan over one or
+many HEOM instances, with the packed state resident in HBM as a torch tensor.
+
+Each instance is a dict with
+    H      (d, d) system Hamiltonian
+    Q      (ncoup, d, d) coupling operators Q_m
+    bath   (etal, etar, etaa, expn, mode) arrays of length K
+    L      hierarchy depth
+    rho0   (d, d) initial reduced density matrix
+    sdip   optional (d, d) system dipole (H(t) = H + sdip f_s(t))
+    cdip   optional (ncoup, d, d) coupling dipoles (Q_m(t) = Q_m + cdip_m f_c(t))
+
+Couplings that are all diagonal are passed as diagonals.  Couplings that are
+not diagonal but commute (including a single coupling operator) are
+diagonalised together: the instance is propagated in that eigenbasis, which
+leaves the hierarchy equations unchanged (every term is a product of
+operators), and results are rotated back.  Couplings that do not commute use
+the dense-coupling kernel path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from .. import _lib
+
+PRECISIONS = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED, "fp32": _lib.PREC_MIXED}
+
+
+def _is_diag(m, tol=0.0):
+    off = m - np.diag(np.diag(m))
+    return np.abs(off).max(initial=0.0) <= tol
+
+
+def common_eigenbasis(mats, tol=1e-10):
+    """Unitary U with U^H M U diagonal for every Hermitian M in mats, or None."""
+    mats = [np.asarray(m, np.complex128) for m in mats]
+    if not all(np.allclose(m, m.conj().T, atol=1e-12) for m in mats):
+        return None
+    rng = np.random.default_rng(12345)
+    comb = sum(rng.uniform(0.5, 1.5) * m for m in mats)
+    _, U = np.linalg.eigh(comb)
+    scale = max(1.0, max(np.abs(m).max() for m in mats))
+    for m in mats:
+        if not _is_diag(U.conj().T @ m @ U, tol * scale):
+            return None
+    return U
+
+
+def prepare_instance(inst):
+    """Choose the working basis of one instance; returns a dict of complex128 arrays ready for the plan."""
+    H = np.asarray(inst["H"], np.complex128)
+    d = H.shape[0]
+    Q = np.asarray(inst["Q"], np.complex128).reshape(-1, d, d)
+    sdip = inst.get("sdip")
+    cdip = inst.get("cdip")
+    sdip = None if sdip is None else np.asarray(sdip, np.complex128)
+    cdip = None if cdip is None else np.asarray(cdip, np.complex128).reshape(-1, d, d)
+    coup = list(Q) + ([] if cdip is None else list(cdip))
+    U = None
+    dense = False
+    if not all(_is_diag(m) for m in coup):
+        U = common_eigenbasis(coup)
+        if U is None:
+            dense = True
+    rot = (lambda m: m) if U is None else (lambda m: U.conj().T @ m @ U)
+    out = dict(d=d, U=U, dense=dense, H=rot(H), sdip=None if sdip is None else rot(sdip),
+               rho0=rot(np.asarray(inst["rho0"], np.complex128)), L=int(inst["L"]))
+    Qw = np.array([rot(q) for q in Q])
+    Cw = None if cdip is None else np.array([rot(c) for c in cdip])
+    out["Qmat"], out["Cmat"] = Qw, Cw
+    out["qdiag"] = np.array([np.diag(q) for q in Qw])
+    out["cdiag"] = None if Cw is None else np.array([np.diag(c) for c in Cw])
+    etal, etar, etaa, expn, mode = inst["bath"]
+    out["bath"] = tuple(np.ascontiguousarray(np.asarray(a, np.complex128)) for a in (etal, etar, etaa, expn))
+    out["mode"] = np.ascontiguousarray(np.asarray(mode, np.int32))
+    return out
+
+
+class HeomEngine:
+    """A plan over one or more HEOM instances plus its device state."""
+
+    def __init__(self, instances, *, resident=True):
+        self.torch = torch = _lib.require_gpu()
+        self.L = _lib.lib()
+        self.prep = [prepare_instance(i) for i in instances]
+        self.n = len(self.prep)
+        self.dense = any(p["dense"] for p in self.prep)
+        keep = []
+        arr = (_lib.HeomInstance * self.n)()
+        for k, p in enumerate(self.prep):
+            d = p["d"]
+            bufs = dict(
+                mode=p["mode"], expn=p["bath"][3], etal=p["bath"][0], etar=p["bath"][1], etaa=p["bath"][2],
+                H=np.ascontiguousarray(p["H"]),
+                sdip=None if p["sdip"] is None else np.ascontiguousarray(p["sdip"]),
+            )
+            if self.dense:
+                bufs["qmat"] = np.ascontiguousarray(p["Qmat"])
+                bufs["cmat"] = None if p["Cmat"] is None else np.ascontiguousarray(p["Cmat"])
+            else:
+                bufs["qdiag"] = np.ascontiguousarray(p["qdiag"])
+                bufs["cdiag"] = None if p["cdiag"] is None else np.ascontiguousarray(p["cdiag"])
+            keep.append(bufs)
+            s = arr[k]
+            s.d, s.K, s.L, s.ncoup = d, len(p["mode"]), p["L"], p["qdiag"].shape[0]
+            for name, b in bufs.items():
+                setattr(s, name, None if b is None else b.ctypes.data)
+        flags = (0 if resident else _lib.PLAN_NO_RESIDENT) | (_lib.PLAN_DENSE_Q if self.dense else 0)
+        plan = C.c_void_p()
+        _lib.check(self.L.pyqed_heom_plan_create(arr, self.n, flags, C.byref(plan)))
+        self.plan = plan
+        ne, na, w64, wmx = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        _lib.check(self.L.pyqed_heom_plan_query(plan, C.byref(ne), C.byref(na), C.byref(w64), C.byref(wmx)))
+        self.n_elems, self.n_ados = ne.value, na.value
+        self.ws_bytes = {_lib.PREC_FP64: w64.value, _lib.PREC_MIXED: wmx.value}
+        so = np.zeros(self.n, np.int64)
+        nm = np.zeros(self.n, np.int64)
+        to = np.zeros(self.n, np.int64)
+        _lib.check(self.L.pyqed_heom_plan_layout(plan, so.ctypes.data, nm.ctypes.data, to.ctypes.data))
+        self.state_offset, self.nmax, self.traj_offset = so, nm, to
+        self.traj_stride = int(sum(p["d"] ** 2 for p in self.prep))
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        self.y = torch.zeros(self.n_elems, dtype=torch.complex128, device=self.device)
+        self._ws = None
+        self._ws_prec = None
+        self.step = 0
+        self.reset()
+
+    # ------------------------------------------------------------------ state
+    def reset(self):
+        """y <- every instance at rho0 in ADO 0, zero elsewhere; time index back to 0."""
+        torch = self.torch
+        self.y.zero_()
+        host = np.zeros(self.traj_stride, np.complex128)
+        for k, p in enumerate(self.prep):
+            host[self.traj_offset[k]:self.traj_offset[k] + p["d"] ** 2] = p["rho0"].reshape(-1)
+        idx = np.concatenate([self.state_offset[k] + np.arange(p["d"] ** 2) for k, p in enumerate(self.prep)])
+        self.y[torch.as_tensor(idx, device=self.device)] = torch.as_tensor(host, device=self.device)
+        self.step = 0
+
+    def workspace(self, prec):
+        torch = self.torch
+        need = self.ws_bytes[prec]
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(max(need, 256), dtype=torch.uint8, device=self.device)
+        return self._ws
+
+    def stats(self):
+        s = np.zeros(4, np.int64)
+        _lib.check(self.L.pyqed_heom_plan_stats(self.plan, s.ctypes.data))
+        return dict(stream_items=int(s[0]), resident_instances=int(s[1]), stream_elems=int(s[2]),
+                    resident_elems=int(s[3]))
+
+    # ------------------------------------------------------------------ stepping
+    def advance(self, nsteps, dt, precision="fp64", field=None, traj=None, save_every=1, stream=None):
+        """Launch nsteps RK4 steps (asynchronous, stream-ordered).  `field` is a device float64 tensor
+        [>= step+nsteps, 6]; `traj` a device complex128 tensor [nsteps//save_every, traj_stride]."""
+        torch = self.torch
+        prec = PRECISIONS[precision]
+        ws = self.workspace(prec)
+        _lib.check(self.L.pyqed_heom_run(
+            self.plan, prec, self.y.data_ptr(), ws.data_ptr(), ws.numel(),
+            None if field is None else field.data_ptr(), self.step, nsteps, float(dt),
+            None if traj is None else traj.data_ptr(), save_every, _lib.stream_ptr(torch, stream)))
+        self.step += nsteps
+
+    def run(self, dt, nt, precision="fp64", field=None, save_every=1):
+        """Propagate nt steps from the current state; returns host trajectories of rho per instance,
+        each [nt//save_every + 1, d, d] (row 0 = the state before the call), in the caller's basis."""
+        torch = self.torch
+        nsave = nt // save_every
+        traj = torch.empty((max(nsave, 1), self.traj_stride), dtype=torch.complex128, device=self.device)
+        first = self.reduced_states()
+        if nt > 0:
+            self.advance(nt, dt, precision, field, traj, save_every)
+        torch.cuda.synchronize()
+        host = traj[:nsave].cpu().numpy()
+        out = []
+        for k, p in enumerate(self.prep):
+            d = p["d"]
+            t = host[:, self.traj_offset[k]:self.traj_offset[k] + d * d].reshape(nsave, d, d)
+            t = np.concatenate([first[k][None], self._unrotate(k, t)], axis=0)
+            out.append(t)
+        return out
+
+    def _unrotate(self, k, rho):
+        U = self.prep[k]["U"]
+        if U is None:
+            return rho
+        return U @ rho @ U.conj().T
+
+    def reduced_states(self):
+        """Current rho (ADO 0) of every instance, host complex128, caller's basis."""
+        out = []
+        for k, p in enumerate(self.prep):
+            d = p["d"]
+            o = int(self.state_offset[k])
+            r = self.y[o:o + d * d].cpu().numpy().reshape(d, d)
+            out.append(self._unrotate(k, r))
+        return out
+
+    def hierarchy(self, k=0):
+        """Full ADO state of instance k as host array [nmax, d, d] (caller's basis)."""
+        p = self.prep[k]
+        d = p["d"]
+        o = int(self.state_offset[k])
+        a = self.y[o:o + int(self.nmax[k]) * d * d].cpu().numpy().reshape(-1, d, d)
+        return self._unrotate(k, a)
+
+    def rhs(self, x, field=None, step=0, col=0):
+        """FP64 right-hand side of a packed state x (device complex128 [n_elems])."""
+        torch = self.torch
+        out = torch.empty_like(x)
+        _lib.check(self.L.pyqed_heom_rhs(self.plan, x.data_ptr(), out.data_ptr(),
+                                         None if field is None else field.data_ptr(), step, col,
+                                         _lib.stream_ptr(torch)))
+        return out
+
+    def tables(self, k=0):
+        K = len(self.prep[k]["mode"])
+        nm = int(self.nmax[k])
+        keys = np.zeros((nm, K), np.int64)
+        plus = np.zeros((nm, K), np.int32)
+        minus = np.zeros((nm, K), np.int32)
+        _lib.check(self.L.pyqed_heom_tables_export(self.plan, k, keys.ctypes.data, plus.ctypes.data,
+                                                   minus.ctypes.data))
+        return keys, plus, minus
+
+    def close(self):
+        if getattr(self, "plan", None):
+            self.L.pyqed_heom_plan_destroy(self.plan)
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def field_table(nt, dt, fs=None, fc=None, step0=0):
+    """Host table [nt, 6] of f_s and f_c at t, t+dt/2, t+dt for t = (step0+i) dt (generate_time's
+    evaluation points, deom.py:730-759)."""
+    tab = np.zeros((step0 + nt, 6), np.float64)
+    for i in range(step0, step0 + nt):
+        t = i * dt
+        for c, tt in enumerate((t, t + dt / 2, t + dt)):
+            for f, col in ((fs, c), (fc, 3 + c)):
+                if f is None:
+                    continue
+                v = complex(f(tt))
+                if v.imag != 0.0:
+                    raise NotImplementedError("complex-valued pulse functions are not supported")
+                tab[i, col] = v.real
+    return tab
