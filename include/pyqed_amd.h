@@ -40,6 +40,7 @@ extern "C" {
 /* Plan flags */
 #define PYQED_PLAN_NO_RESIDENT 1 /* never run instances LDS-resident (always stream through HBM) */
 #define PYQED_PLAN_DENSE_Q 2     /* coupling operators given as dense d x d matrices (qmat) */
+#define PYQED_PLAN_GENERIC 4     /* do not use the site-projector kernels (for cross-checking) */
 
 /* Thread-local description of the last error (never NULL). */
 const char* pyqed_last_error(void);
@@ -137,6 +138,10 @@ int pyqed_heom_rhs(pyqed_heom_plan* plan, const void* x, void* out, const double
  * streamed work items, resident instances, streamed elements, resident
  * elements. */
 int pyqed_heom_plan_stats(const pyqed_heom_plan* plan, int64_t* stats4);
+/* Streamed work per kernel class (0 generic, 1 site-projector with VALU
+ * commutator, 2 site-projector with MFMA commutator): items and elements
+ * (host arrays [3], either may be NULL). */
+int pyqed_heom_plan_classes(const pyqed_heom_plan* plan, int64_t* items, int64_t* elems);
 
 /* ------------------------------------------------------------------ Liouville-space RK4
  * Batched propagation of vec(rho) (row-major, length N = d*d) under

@@ -15,12 +15,13 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 PLAN_NO_RESIDENT = 1
 PLAN_DENSE_Q = 2
+PLAN_GENERIC = 4
 
 SYMBOLS = [
     "pyqed_last_error", "pyqed_version", "pyqed_device_count",
     "pyqed_heom_plan_create", "pyqed_heom_plan_destroy", "pyqed_heom_plan_query", "pyqed_heom_plan_layout",
     "pyqed_heom_tables_export", "pyqed_heom_tables_size", "pyqed_heom_tables_build", "pyqed_heom_run", "pyqed_heom_rhs", "pyqed_heom_plan_stats",
-    "pyqed_liouville_run", "pyqed_liouville_adjoint",
+    "pyqed_heom_plan_classes", "pyqed_liouville_run", "pyqed_liouville_adjoint",
 ]
 
 
@@ -67,6 +68,7 @@ def lib():
     L.pyqed_heom_run.argtypes = [vp, i32, vp, vp, i64, vp, i64, i64, dbl, vp, i64, vp]
     L.pyqed_heom_rhs.argtypes = [vp, vp, vp, vp, i64, i32, vp]
     L.pyqed_heom_plan_stats.argtypes = [vp, vp]
+    L.pyqed_heom_plan_classes.argtypes = [vp, vp, vp]
     L.pyqed_liouville_run.argtypes = [i32, i32, vp, i32, vp, vp, i64, dbl, i32, vp, vp, vp, i64, vp]
     L.pyqed_liouville_adjoint.argtypes = [i32, i32, vp, i32, vp, vp, i64, dbl, i32, vp, vp, vp, vp, vp, i64, vp]
     for name in SYMBOLS:

@@ -30,6 +30,10 @@
 #include <complex>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -53,6 +57,10 @@ struct InstDev {
   int64_t traj_off;
   int32_t rowptr_off, ent_off, coef_off, mode_off;
   int32_t h_off, q_off;  // offsets (complex elements) into the matrix pool
+  // site-projector path: site-ordered tables and coefficients scaled by the projector weight
+  int32_t path;          // 0 generic, 1 projector/VALU commutator, 2 projector/MFMA commutator
+  int32_t p_rowptr_off, p_ent_off, coefp_off;
+  int64_t p_gs_off;      // byte offset of this instance's site-group starts [nmax][d+1]
 };
 
 struct WorkItem {
@@ -70,6 +78,10 @@ struct StageParams {
   const int32_t* mode;
   const double2* mat;   // H, sdip, qdiag/qmat, cdiag/cmat
   const int32_t* resident;  // resident instance ids (resident kernel)
+  const int32_t* p_rowptr;  // site-ordered tables (projector path)
+  const int2* p_ent;
+  const uint8_t* p_gs;
+  const double2* coefp;
   const double* field;
   int64_t step;
   double dt;
@@ -256,6 +268,47 @@ __device__ __forceinline__ cpx<T> element_rhs(const StageParams& p, const InstDe
   return r;
 }
 
+// RK4 stage update of element g (absolute index) from its right-hand side (rx, ry).
+// `save` marks ADO 0 of an instance on a trajectory step; tpos is its slot index.
+template <typename TO, int STAGE>
+__device__ __forceinline__ void rk4_apply(const StageParams& p, int64_t g, double rx, double ry,
+                                          cpx_store<TO>* xout, bool save, int64_t tpos) {
+  const double dt = p.dt;
+  if (STAGE == 4) {
+    ((double2*)p.xout)[g] = make_double2(rx, ry);
+  } else if (STAGE == 0) {
+    const double2 yv = p.y[g];
+    p.ynew[g] = make_double2(yv.x + rx * (dt / 6), yv.y + ry * (dt / 6));
+    st(xout + g, mk<TO>((TO)(yv.x + rx * (dt / 2)), (TO)(yv.y + ry * (dt / 2))));
+  } else if (STAGE == 1 || STAGE == 2) {
+    const double2 yv = p.y[g];
+    const double2 an = p.ynew[g];
+    p.ynew[g] = make_double2(an.x + rx * (dt / 3), an.y + ry * (dt / 3));
+    const double h = STAGE == 1 ? dt / 2 : dt;
+    st(xout + g, mk<TO>((TO)(yv.x + rx * h), (TO)(yv.y + ry * h)));
+  } else {
+    const double2 an = p.ynew[g];
+    const double2 yn = make_double2(an.x + rx * (dt / 6), an.y + ry * (dt / 6));
+    p.y[g] = yn;
+    if (save) p.traj[tpos] = yn;
+  }
+}
+
+__device__ __forceinline__ void item_range(const StageParams& p, int& it0, int& itend, int& itstep) {
+  if (p.use_xcd) {
+    // blocks are dispatched round-robin over the 8 XCDs: block b runs on XCD b % 8, and the blocks of
+    // one XCD sweep one contiguous eighth of the queue so an instance's tiles share that XCD's L2
+    const int x = blockIdx.x & 7, j = blockIdx.x >> 3, nbx = gridDim.x >> 3;
+    it0 = p.item_begin + p.xr[x] + j;
+    itend = p.item_begin + p.xr[x + 1];
+    itstep = nbx;
+  } else {
+    it0 = p.item_begin + blockIdx.x;
+    itend = p.item_end;
+    itstep = gridDim.x;
+  }
+}
+
 // ------------------------------------------------------------------ streamed stage kernel
 // STAGE 0..3: RK4 stage; STAGE 4: out = RHS(xin) only (FP64 operator check).
 template <typename T, typename TI, typename TO, int STAGE>
@@ -266,22 +319,12 @@ __global__ __launch_bounds__(kThreads) void heom_stage_kernel(StageParams p) {
   const int tid = threadIdx.x;
   const cpx_store<TI>* xin = (const cpx_store<TI>*)p.xin;
   cpx_store<TO>* xout = (cpx_store<TO>*)p.xout;
-  const double dt = p.dt;
   const int col = p.col_override >= 0 ? p.col_override : stage_col(STAGE & 3);
   double fc = 0.0;
   if (p.field) fc = p.field[6 * p.step + 3 + col];
 
   int it0, itstep, itend;
-  if (p.use_xcd) {
-    const int x = blockIdx.x & 7, j = blockIdx.x >> 3, nbx = gridDim.x >> 3;
-    it0 = p.item_begin + p.xr[x] + j;
-    itend = p.item_begin + p.xr[x + 1];
-    itstep = nbx;
-  } else {
-    it0 = p.item_begin + blockIdx.x;
-    itend = p.item_end;
-    itstep = gridDim.x;
-  }
+  item_range(p, it0, itend, itstep);
   for (int it = it0; it < itend; it += itstep) {
     const WorkItem w = p.items[it];
     const InstDev I = p.insts[w.inst];
@@ -304,27 +347,215 @@ __global__ __launch_bounds__(kThreads) void heom_stage_kernel(StageParams p) {
           const int i = e / d, j = e - (e / d) * d;
           const cpx<T> r = element_rhs<T>(p, I, l, own, xi, a, i, j, fc);
           const int64_t g = I.off + (int64_t)a * d2 + e;
-          const double rx = (double)r.x, ry = (double)r.y;
-          if (STAGE == 4) {
-            ((double2*)p.xout)[g] = make_double2(rx, ry);
-          } else if (STAGE == 0) {
-            const double2 yv = p.y[g];
-            p.ynew[g] = make_double2(yv.x + rx * (dt / 6), yv.y + ry * (dt / 6));
-            st(xout + g, mk<TO>((TO)(yv.x + rx * (dt / 2)), (TO)(yv.y + ry * (dt / 2))));
-          } else if (STAGE == 1 || STAGE == 2) {
-            const double2 yv = p.y[g];
-            const double2 an = p.ynew[g];
-            p.ynew[g] = make_double2(an.x + rx * (dt / 3), an.y + ry * (dt / 3));
-            const double h = STAGE == 1 ? dt / 2 : dt;
-            st(xout + g, mk<TO>((TO)(yv.x + rx * h), (TO)(yv.y + ry * h)));
-          } else {
-            const double2 an = p.ynew[g];
-            const double2 yn = make_double2(an.x + rx * (dt / 6), an.y + ry * (dt / 6));
-            p.y[g] = yn;
-            if (a == 0 && p.traj && p.save_slot >= 0) p.traj[p.save_slot * p.traj_stride + I.traj_off + e] = yn;
-          }
+          const bool save = STAGE == 3 && a == 0 && p.traj && p.save_slot >= 0;
+          rk4_apply<TO, STAGE>(p, g, (double)r.x, (double)r.y, xout, save,
+                               p.save_slot * p.traj_stride + I.traj_off + e);
         }
       }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ site-projector stage kernel
+// For couplings Q_m = c_m |p_m><p_m| the coupling term of element (i, j) of ADO n is
+//   sum_{k: site(k) = i} c cL_k X'_k(i, j) + sum_{k: site(k) = j} c cR_k X'_k(i, j),
+// i.e. only the neighbours coupled through site i or site j contribute, each at the element's own
+// position.  The tables list every ADO's neighbours grouped by site (gs[a][s] = first entry of site
+// s), so each element visits only its two groups.  One wavefront owns one ADO: a prologue stages
+// the ADO's neighbour list with its coefficients in LDS and reduces the damping term; the
+// commutator runs on VALU (d <= 11, d >= 17) or on FP32 MFMA 16x16x4 (12 <= d <= 16, padded to 16).
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMaxEnt = 64;  // neighbour entries per ADO handled by the projector path
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> wave_sum(cpx<T> v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    v.x += __shfl_xor(v.x, o, 64);
+    v.y += __shfl_xor(v.y, o, 64);
+  }
+  return v;
+}
+
+template <typename T>
+struct WaveLds {
+  cpx<T>* X;    // own ADO, d*d
+  int32_t* nb;  // neighbour ADO index per entry (site-ordered)
+  cpx<T>* cl;   // coefficient of the row term
+  cpx<T>* cr;   // coefficient of the column term
+  int32_t* gs;  // d+1 site-group starts
+};
+
+__host__ __device__ inline size_t proj_wave_bytes(int d2max, size_t ts) {
+  size_t b = 2 * ts * (size_t)d2max + 4 * kMaxEnt + 2 * 2 * ts * kMaxEnt + 4 * 40;
+  return (b + 15) & ~size_t(15);
+}
+__host__ __device__ inline size_t proj_block_bytes(int d2max, int Kmax, int Lmax, size_t ts) {
+  size_t b = 2 * ts * (size_t)d2max + 2 * ts * 5 * (size_t)Kmax + ts * (size_t)(Lmax + 2);
+  return (b + 15) & ~size_t(15);
+}
+
+template <typename T, typename TI, typename TO, int STAGE, bool MFMA>
+__global__ __launch_bounds__(kThreads) void heom_proj_kernel(StageParams p) {
+  extern __shared__ __align__(16) char smem[];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  cpx<T>* sH = (cpx<T>*)smem;
+  cpx<T>* sC = sH + p.d2max;
+  T* sS = (T*)(sC + 5 * p.Kmax);
+  char* wbase = smem + proj_block_bytes(p.d2max, p.Kmax, p.Lmax, sizeof(T));
+  WaveLds<T> w;
+  {
+    char* b = wbase + wv * proj_wave_bytes(p.d2max, sizeof(T));
+    w.X = (cpx<T>*)b;
+    w.cl = w.X + p.d2max;
+    w.cr = w.cl + kMaxEnt;
+    w.nb = (int32_t*)(w.cr + kMaxEnt);
+    w.gs = w.nb + kMaxEnt;
+  }
+  const cpx_store<TI>* xin = (const cpx_store<TI>*)p.xin;
+  cpx_store<TO>* xout = (cpx_store<TO>*)p.xout;
+  const int col = p.col_override >= 0 ? p.col_override : stage_col(STAGE & 3);
+  int it0, itstep, itend;
+  item_range(p, it0, itend, itstep);
+  for (int it = it0; it < itend; it += itstep) {
+    const WorkItem wi = p.items[it];
+    const InstDev I = p.insts[wi.inst];
+    const int d = I.d, d2 = I.d2;
+    __syncthreads();
+    {
+      double fs = 0.0;
+      if (p.field) fs = p.field[6 * p.step + col];
+      const double2* H = p.mat + I.h_off;
+      const bool sd = (I.flags & F_SDIP) != 0;
+      for (int e = tid; e < d2; e += kThreads) {
+        double2 h = H[e];
+        if (sd) {
+          const double2 s2 = H[d2 + e];
+          h.x += s2.x * fs;
+          h.y += s2.y * fs;
+        }
+        sH[e] = mk<T>((T)h.x, (T)h.y);
+      }
+      for (int e = tid; e < 5 * I.K; e += kThreads) {
+        const double2 v = p.coefp[I.coefp_off + e];
+        sC[e] = mk<T>((T)v.x, (T)v.y);
+      }
+      for (int e = tid; e < I.L + 2; e += kThreads) sS[e] = (T)sqrt((double)e);
+    }
+    __syncthreads();
+    // H fragments for the MFMA commutator (lane = (gk, gi), gk = lane / 16, gi = lane % 16)
+    const int gi = lane & 15, gk = lane >> 4;
+    float hAr[4], hAi[4], nhAi[4], hBi[4], nhBr[4], nhBi[4];
+    if (MFMA) {
+#pragma unroll
+      for (int kc = 0; kc < 4; ++kc) {
+        const int k = 4 * kc + gk;
+        const bool ok = gi < d && k < d;
+        const cpx<T> a = ok ? sH[gi * d + k] : mk<T>(0, 0);  // A-layout H[gi][k]
+        const cpx<T> b = ok ? sH[k * d + gi] : mk<T>(0, 0);  // B-layout H[k][gi]
+        hAr[kc] = (float)a.x;
+        hAi[kc] = (float)a.y;
+        nhAi[kc] = -(float)a.y;
+        hBi[kc] = (float)b.y;
+        nhBr[kc] = -(float)b.x;
+        nhBi[kc] = -(float)b.y;
+      }
+    }
+    const cpx_store<TI>* xi = xin + I.off;
+    for (int a = wi.ado0 + wv; a < wi.ado0 + wi.nado; a += kThreads / 64) {
+      const int64_t base = (int64_t)a * d2;
+      // ---- prologue: neighbour list with coefficients, damping, site groups, own matrix
+      const int rp0 = p.p_rowptr[I.p_rowptr_off + a];
+      const int nent = p.p_rowptr[I.p_rowptr_off + a + 1] - rp0;
+      cpx<T> dp = mk<T>(0, 0);
+      if (lane < nent) {
+        const int2 en = p.p_ent[I.p_ent_off + rp0 + lane];
+        const int k = en.y & 0xffff, nf = (en.y >> 16) & 0x7f;
+        const bool plus = en.y < 0;
+        const cpx<T>* C = sC + 5 * k;
+        const T sq = sS[nf];
+        w.nb[lane] = en.x;
+        w.cl[lane] = (plus ? C[3] : C[1]) * sq;
+        w.cr[lane] = (plus ? C[4] : C[2]) * sq;
+        if (!plus) dp = C[0] * (T)nf;
+      }
+      if (lane <= d) w.gs[lane] = p.p_gs[I.p_gs_off + (int64_t)a * (d + 1) + lane];
+      for (int e = lane; e < d2; e += 64) w.X[e] = ldc<T>(xi, base + e);
+      const cpx<T> damp = wave_sum(dp);
+      wave_sync();
+      const bool save_ado = STAGE == 3 && a == 0 && p.traj && p.save_slot >= 0;
+      const int64_t tbase = p.save_slot * p.traj_stride + I.traj_off;
+      if (MFMA) {
+        floatx4 aR = {0.f, 0.f, 0.f, 0.f}, aI = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc) {
+          const int k = 4 * kc + gk;
+          const bool ok = gi < d && k < d;
+          const cpx<T> xb = ok ? w.X[k * d + gi] : mk<T>(0, 0);  // B-layout X[k][gi]
+          const cpx<T> xa = ok ? w.X[gi * d + k] : mk<T>(0, 0);  // A-layout X[gi][k]
+          const float xbr = (float)xb.x, xbi = (float)xb.y, xar = (float)xa.x, xai = (float)xa.y;
+          // + H X
+          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(hAr[kc], xbr, aR, 0, 0, 0);
+          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(nhAi[kc], xbi, aR, 0, 0, 0);
+          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(hAr[kc], xbi, aI, 0, 0, 0);
+          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(hAi[kc], xbr, aI, 0, 0, 0);
+          // - X H
+          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(xar, nhBr[kc], aR, 0, 0, 0);
+          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(xai, hBi[kc], aR, 0, 0, 0);
+          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(xar, nhBi[kc], aI, 0, 0, 0);
+          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(xai, nhBr[kc], aI, 0, 0, 0);
+        }
+        // elements (4 gk + r, gi), r = 0..3
+        cpx<T> acc[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 4 * gk + r;
+          acc[r] = mul_mi(mk<T>((T)aR[r], (T)aI[r]));
+          if (i < d && gi < d) cmsub(acc[r], damp, w.X[i * d + gi]);
+        }
+        if (gi < d) {
+          for (int t = w.gs[gi]; t < w.gs[gi + 1]; ++t) {
+            const int64_t nb = (int64_t)w.nb[t] * d2 + gi;
+            const cpx<T> c = w.cr[t];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              if (4 * gk + r < d) cmac(acc[r], c, ldc<T>(xi, nb + (4 * gk + r) * d));
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = 4 * gk + r;
+            if (i < d) {
+              for (int t = w.gs[i]; t < w.gs[i + 1]; ++t)
+                cmac(acc[r], w.cl[t], ldc<T>(xi, (int64_t)w.nb[t] * d2 + i * d + gi));
+              const int e = i * d + gi;
+              rk4_apply<TO, STAGE>(p, I.off + base + e, (double)acc[r].x, (double)acc[r].y, xout, save_ado,
+                                   tbase + e);
+            }
+          }
+        }
+      } else {
+        for (int e = lane; e < d2; e += 64) {
+          const int i = e / d, j = e - (e / d) * d;
+          cpx<T> c = mk<T>(0, 0);
+          for (int k = 0; k < d; ++k) {
+            cmac(c, sH[i * d + k], w.X[k * d + j]);
+            cmsub(c, w.X[i * d + k], sH[k * d + j]);
+          }
+          cpx<T> r = mul_mi(c);
+          cmsub(r, damp, w.X[e]);
+          for (int t = w.gs[i]; t < w.gs[i + 1]; ++t) cmac(r, w.cl[t], ldc<T>(xi, (int64_t)w.nb[t] * d2 + e));
+          for (int t = w.gs[j]; t < w.gs[j + 1]; ++t) cmac(r, w.cr[t], ldc<T>(xi, (int64_t)w.nb[t] * d2 + e));
+          rk4_apply<TO, STAGE>(p, I.off + base + e, (double)r.x, (double)r.y, xout, save_ado, tbase + e);
+        }
+      }
+      wave_sync();
     }
   }
 }
@@ -445,15 +676,22 @@ int build_tables(int K, int L, Tables& t) {
   t.keys.assign((size_t)t.nmax * K, 0);
   t.rowptr.assign((size_t)t.nmax + 1, 0);
   t.ent.clear();
-  std::vector<int64_t> s(K), cvec(K + 1);
+  std::vector<int64_t> s(K), cvec(K + 1), up(K + 1, 0), dn(K + 1, 0);
+  t.ent.reserve((size_t)t.nmax * (size_t)std::max(1, std::min(K, L)) * 2);
   for (int r = 0; r < t.nmax; ++r) {
     // unrank: N = sum_{j=1..K} C(c_j, j), c_1 < c_2 < ... < c_K
     int64_t N = r;
+    int hi = K + L;  // c_K <= K + L - 1 and c_j < c_{j+1}
     for (int j = K; j >= 1; --j) {
-      int c = j - 1;
-      while (C.get(c + 1, j) <= N) ++c;
-      N -= C.get(c, j);
-      cvec[j] = c;
+      // largest c in [j-1, hi] with C(c, j) <= N (C(c, j) is nondecreasing in c)
+      int lo = j - 1, top = hi;
+      while (lo < top) {
+        const int mid = (lo + top + 1) / 2;
+        if (C.get(mid, j) <= N) lo = mid; else top = mid - 1;
+      }
+      N -= C.get(lo, j);
+      cvec[j] = lo;
+      hi = lo - 1;
     }
     int tier = 0;
     for (int i = 0; i < K; ++i) {
@@ -463,7 +701,7 @@ int build_tables(int K, int L, Tables& t) {
     }
     tier = K ? (int)s[K - 1] : 0;
     // suffix sums for neighbour ranks
-    std::vector<int64_t> up(K + 1, 0), dn(K + 1, 0);
+    up[K] = dn[K] = 0;
     for (int i = K - 1; i >= 0; --i) {
       up[i] = up[i + 1] + C.get((int)s[i] + i, i);
       dn[i] = dn[i + 1] + C.get((int)s[i] + i - 1, i);
@@ -489,6 +727,71 @@ int build_tables(int K, int L, Tables& t) {
   return PYQED_OK;
 }
 
+// Process-wide cache of built tables keyed by (K, L): many plans share few shapes.
+int cached_tables(int K, int L, std::shared_ptr<const Tables>& out) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, std::shared_ptr<const Tables>> cache;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find({K, L});
+    if (it != cache.end()) {
+      out = it->second;
+      return PYQED_OK;
+    }
+  }
+  auto t = std::make_shared<Tables>();
+  int rc = build_tables(K, L, *t);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(mu);
+  auto& slot = cache[{K, L}];
+  if (!slot) slot = t;
+  out = slot;
+  return PYQED_OK;
+}
+
+// Site-ordered copy of a (K, L) table for the projector path: every ADO's entries sorted (stably)
+// by the site their exponential couples to, plus gs[a][s] = number of the ADO's entries with
+// site < s (s = 0..d).
+struct ProjTables {
+  std::vector<int32_t> rowptr;
+  std::vector<int2> ent;
+  std::vector<uint8_t> gs;  // [nmax][d+1]
+  int max_ent = 0;
+};
+
+int cached_proj_tables(const Tables& t, int d, const std::vector<int>& site, std::shared_ptr<const ProjTables>& out) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, std::vector<int>>, std::shared_ptr<const ProjTables>> cache;
+  const auto key = std::make_tuple(t.K, t.L, d, site);
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      out = it->second;
+      return PYQED_OK;
+    }
+  }
+  auto pt = std::make_shared<ProjTables>();
+  pt->rowptr = t.rowptr;
+  pt->ent.resize(t.ent.size());
+  pt->gs.assign((size_t)t.nmax * (d + 1), 0);
+  std::vector<int> cnt(d + 1);
+  for (int a = 0; a < t.nmax; ++a) {
+    const int r0 = t.rowptr[a], r1 = t.rowptr[a + 1];
+    pt->max_ent = std::max(pt->max_ent, r1 - r0);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int q = r0; q < r1; ++q) cnt[site[t.ent[q].y & 0xffff] + 1]++;
+    for (int s2 = 0; s2 < d; ++s2) cnt[s2 + 1] += cnt[s2];
+    for (int s2 = 0; s2 <= d; ++s2) pt->gs[(size_t)a * (d + 1) + s2] = (uint8_t)std::min(cnt[s2], 255);
+    for (int q = r0; q < r1; ++q) pt->ent[r0 + cnt[site[t.ent[q].y & 0xffff]]++] = t.ent[q];
+  }
+  std::lock_guard<std::mutex> g(mu);
+  auto& slot = cache[key];
+  if (!slot) slot = pt;
+  out = slot;
+  return PYQED_OK;
+}
+
 template <typename K>
 void* kernel_ptr(K k) {
   return reinterpret_cast<void*>(k);
@@ -500,13 +803,15 @@ struct pyqed_heom_plan {
   int n_inst = 0, flags = 0, device = 0;
   std::vector<InstDev> inst;
   std::vector<int64_t> nmax;
-  std::map<std::pair<int, int>, Tables> tabs;
+  std::map<std::pair<int, int>, std::shared_ptr<const Tables>> tabs;
+  std::vector<std::shared_ptr<const ProjTables>> ptabs;
   std::vector<int> inst_K, inst_L;
   int64_t n_elems = 0, n_ados = 0, traj_stride = 0;
   int d2max = 1, qmax = 1, Kmax = 1, Lmax = 0;
-  std::vector<WorkItem> items;  // streamed items, then items of resident instances
+  std::vector<WorkItem> items;  // streamed items by class (generic, projector/VALU, projector/MFMA), then resident
+  int cls_begin[4] = {0, 0, 0, 0};  // class c occupies [cls_begin[c], cls_begin[c+1])
   int n_items_stream = 0;
-  int xr[9] = {0};
+  int xr[3][9] = {{0}};
   std::vector<int32_t> resident;
   int64_t res_elems = 0, stream_elems = 0;
   int num_cu = 256, max_lds = 65536;
@@ -519,6 +824,11 @@ struct pyqed_heom_plan {
   int32_t* d_mode = nullptr;
   double2* d_mat = nullptr;
   int32_t* d_res = nullptr;
+  int32_t* d_prowptr = nullptr;
+  int2* d_pent = nullptr;
+  uint8_t* d_pgs = nullptr;
+  double2* d_coefp = nullptr;
+  int n_proj = 0;
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
@@ -542,6 +852,10 @@ size_t stream_lds_bytes(const pyqed_heom_plan* P, size_t tsize) {
   return h + x;
 }
 
+size_t proj_lds_bytes(const pyqed_heom_plan* P, size_t tsize) {
+  return proj_block_bytes(P->d2max, P->Kmax, P->Lmax, tsize) + (kThreads / 64) * proj_wave_bytes(P->d2max, tsize);
+}
+
 size_t resident_lds_bytes(const pyqed_heom_plan* P, int64_t n_el, size_t tsize) {
   size_t h = lds_header_bytes(P->d2max, P->qmax, P->Kmax, P->Lmax, tsize);
   return h + (size_t)n_el * (32 + 2 * 2 * tsize);
@@ -563,6 +877,10 @@ void destroy_plan(pyqed_heom_plan* P) {
   (void)hipFree(P->d_mode);
   (void)hipFree(P->d_mat);
   (void)hipFree(P->d_res);
+  (void)hipFree(P->d_prowptr);
+  (void)hipFree(P->d_pent);
+  (void)hipFree(P->d_pgs);
+  (void)hipFree(P->d_coefp);
   if (P->side) (void)hipStreamDestroy(P->side);
   if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
   if (P->ev_join) (void)hipEventDestroy(P->ev_join);
@@ -611,13 +929,13 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     }
     auto key = std::make_pair((int)s.K, (int)s.L);
     if (!P->tabs.count(key)) {
-      Tables t;
-      int rc = build_tables(s.K, s.L, t);
+      std::shared_ptr<const Tables> t;
+      int rc = cached_tables(s.K, s.L, t);
       if (rc) {
         destroy_plan(P);
         return rc;
       }
-      P->tabs.emplace(key, std::move(t));
+      P->tabs.emplace(key, t);
     }
   }
   // ---- pack pools
@@ -626,15 +944,18 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
   std::map<std::pair<int, int>, std::pair<int, int>> tab_off;
   for (auto& kv : P->tabs) {
     tab_off[kv.first] = {(int)rowptr.size(), (int)ent.size()};
-    rowptr.insert(rowptr.end(), kv.second.rowptr.begin(), kv.second.rowptr.end());
-    ent.insert(ent.end(), kv.second.ent.begin(), kv.second.ent.end());
+    rowptr.insert(rowptr.end(), kv.second->rowptr.begin(), kv.second->rowptr.end());
+    ent.insert(ent.end(), kv.second->ent.begin(), kv.second->ent.end());
   }
-  std::vector<double2> coef, mat;
-  std::vector<int32_t> mode;
+  std::vector<double2> coef, mat, coefp;
+  std::vector<int32_t> mode, prowptr;
+  std::vector<int2> pent;
+  std::vector<uint8_t> pgs;
+  std::map<const ProjTables*, std::tuple<int, int, int64_t>> ptab_off;
   int64_t off = 0, toff = 0, ados = 0;
   for (int i = 0; i < n_inst; ++i) {
     const pyqed_heom_instance& s = in[i];
-    const Tables& t = P->tabs.at({s.K, s.L});
+    const Tables& t = *P->tabs.at({s.K, s.L});
     InstDev I;
     std::memset(&I, 0, sizeof(I));
     I.d = s.d;
@@ -670,7 +991,61 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     for (int e = 0; e < nq; ++e) mat.push_back(make_double2(qsrc[2 * e], qsrc[2 * e + 1]));
     if (csrc)
       for (int e = 0; e < nq; ++e) mat.push_back(make_double2(csrc[2 * e], csrc[2 * e + 1]));
-    if (mat.size() > (size_t)INT32_MAX || coef.size() > (size_t)INT32_MAX) {
+    // ---- site-projector path: every coupling is c_m |p_m><p_m| (diagonal with one nonzero entry)
+    I.path = 0;
+    if (!dense && !s.cdiag && s.d >= 6 && !(flags & PYQED_PLAN_GENERIC)) {
+      std::vector<int> psite(s.ncoup, -1);
+      std::vector<std::complex<double>> pw(s.ncoup);
+      bool proj = true;
+      for (int m = 0; m < s.ncoup && proj; ++m) {
+        int nz = 0;
+        for (int e = 0; e < s.d; ++e) {
+          const std::complex<double> q = cz(s.qdiag, m * s.d + e);
+          if (q != 0.0) {
+            ++nz;
+            psite[m] = e;
+            pw[m] = q;
+          }
+        }
+        proj = nz == 1;
+      }
+      if (proj) {
+        std::vector<int> site(s.K);
+        for (int k = 0; k < s.K; ++k) site[k] = psite[s.mode[k]];
+        std::shared_ptr<const ProjTables> pt;
+        int rc = cached_proj_tables(t, s.d, site, pt);
+        if (rc) {
+          destroy_plan(P);
+          return rc;
+        }
+        if (pt->max_ent <= kMaxEnt) {
+          P->ptabs.push_back(pt);
+          I.path = (s.d >= 12 && s.d <= 16) ? 2 : 1;
+          if (!ptab_off.count(pt.get())) {
+            ptab_off[pt.get()] = std::make_tuple((int)prowptr.size(), (int)pent.size(), (int64_t)pgs.size());
+            prowptr.insert(prowptr.end(), pt->rowptr.begin(), pt->rowptr.end());
+            pent.insert(pent.end(), pt->ent.begin(), pt->ent.end());
+            pgs.insert(pgs.end(), pt->gs.begin(), pt->gs.end());
+          }
+          const auto& o = ptab_off[pt.get()];
+          I.p_rowptr_off = std::get<0>(o);
+          I.p_ent_off = std::get<1>(o);
+          I.p_gs_off = std::get<2>(o);
+          I.coefp_off = (int)coefp.size();
+          for (int k = 0; k < s.K; ++k) {
+            const std::complex<double> wgt = pw[s.mode[k]];
+            for (int q = 0; q < 5; ++q) {
+              const double2 c = coef[I.coef_off + 5 * k + q];
+              std::complex<double> v(c.x, c.y);
+              if (q > 0) v *= wgt;
+              coefp.push_back(make_double2(v.real(), v.imag()));
+            }
+          }
+          P->n_proj++;
+        }
+      }
+    }
+    if (mat.size() > (size_t)INT32_MAX || coef.size() > (size_t)INT32_MAX || pent.size() > (size_t)INT32_MAX) {
       destroy_plan(P);
       return fail(PYQED_EINVAL, "coefficient pools exceed 2^31 entries");
     }
@@ -713,7 +1088,7 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
       std::min<int64_t>(8192, std::max<int64_t>(256, P->stream_elems / std::max<int64_t>(1, (int64_t)P->num_cu * 8)));
   auto tile = [&](int i) {
     const InstDev& I = P->inst[i];
-    const int nslot = kThreads / I.grp;
+    const int nslot = (I.path == 0 || is_res[i]) ? kThreads / I.grp : kThreads / 64;
     int64_t per = (target + I.d2 - 1) / I.d2;
     per = ((per + nslot - 1) / nslot) * nslot;
     per = std::max<int64_t>(per, nslot);
@@ -726,28 +1101,32 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
       P->items.push_back(w);
     }
   };
-  for (int i = 0; i < n_inst; ++i)
-    if (!is_res[i]) tile(i);
-  P->n_items_stream = (int)P->items.size();
+  for (int c = 0; c < 3; ++c) {
+    P->cls_begin[c] = (int)P->items.size();
+    for (int i = 0; i < n_inst; ++i)
+      if (!is_res[i] && P->inst[i].path == c) tile(i);
+  }
+  P->cls_begin[3] = P->n_items_stream = (int)P->items.size();
   for (int i = 0; i < n_inst; ++i)
     if (is_res[i]) {
       tile(i);
       P->resident.push_back(i);
     }
-  // XCD ranges over the streamed items, balanced by element count
-  {
-    std::vector<int64_t> pre(P->n_items_stream + 1, 0);
-    for (int k = 0; k < P->n_items_stream; ++k)
-      pre[k + 1] = pre[k] + (int64_t)P->items[k].nado * P->inst[P->items[k].inst].d2;
-    P->xr[0] = 0;
+  // XCD ranges of each class, balanced by element count
+  for (int c = 0; c < 3; ++c) {
+    const int b0 = P->cls_begin[c], n = P->cls_begin[c + 1] - b0;
+    std::vector<int64_t> pre(n + 1, 0);
+    for (int k = 0; k < n; ++k) pre[k + 1] = pre[k] + (int64_t)P->items[b0 + k].nado * P->inst[P->items[b0 + k].inst].d2;
+    P->xr[c][0] = 0;
     for (int x = 1; x < 8; ++x) {
       const int64_t goal = pre.back() * x / 8;
-      P->xr[x] = (int)(std::lower_bound(pre.begin(), pre.end(), goal) - pre.begin());
-      P->xr[x] = std::max(P->xr[x], P->xr[x - 1]);
+      P->xr[c][x] = (int)(std::lower_bound(pre.begin(), pre.end(), goal) - pre.begin());
+      P->xr[c][x] = std::max(P->xr[c][x], P->xr[c][x - 1]);
     }
-    P->xr[8] = P->n_items_stream;
+    P->xr[c][8] = n;
   }
-  if (stream_lds_bytes(P, sizeof(double)) > (size_t)P->max_lds) {
+  if (stream_lds_bytes(P, sizeof(double)) > (size_t)P->max_lds ||
+      proj_lds_bytes(P, sizeof(double)) > (size_t)P->max_lds) {
     destroy_plan(P);
     return fail(PYQED_EUNSUPPORTED, "instance too large for the streamed kernel's LDS budget");
   }
@@ -755,7 +1134,9 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
   int rc = 0;
   if ((rc = upload(P->inst, &P->d_inst)) || (rc = upload(P->items, &P->d_items)) ||
       (rc = upload(rowptr, &P->d_rowptr)) || (rc = upload(ent, &P->d_ent)) || (rc = upload(coef, &P->d_coef)) ||
-      (rc = upload(mode, &P->d_mode)) || (rc = upload(mat, &P->d_mat)) || (rc = upload(P->resident, &P->d_res))) {
+      (rc = upload(mode, &P->d_mode)) || (rc = upload(mat, &P->d_mat)) || (rc = upload(P->resident, &P->d_res)) ||
+      (rc = upload(prowptr, &P->d_prowptr)) || (rc = upload(pent, &P->d_pent)) || (rc = upload(pgs, &P->d_pgs)) ||
+      (rc = upload(coefp, &P->d_coefp))) {
     destroy_plan(P);
     return rc;
   }
@@ -803,6 +1184,19 @@ extern "C" int pyqed_heom_plan_stats(const pyqed_heom_plan* P, int64_t* s) {
   return PYQED_OK;
 }
 
+extern "C" int pyqed_heom_plan_classes(const pyqed_heom_plan* P, int64_t* items, int64_t* elems) {
+  if (!P) return fail(PYQED_EINVAL, "plan_classes: null plan");
+  for (int c = 0; c < 3; ++c) {
+    if (items) items[c] = P->cls_begin[c + 1] - P->cls_begin[c];
+    if (elems) {
+      elems[c] = 0;
+      for (int k = P->cls_begin[c]; k < P->cls_begin[c + 1]; ++k)
+        elems[c] += (int64_t)P->items[k].nado * P->inst[P->items[k].inst].d2;
+    }
+  }
+  return PYQED_OK;
+}
+
 namespace {
 
 void export_tables(const Tables& t, int64_t* keys, int32_t* plus, int32_t* minus) {
@@ -827,7 +1221,7 @@ void export_tables(const Tables& t, int64_t* keys, int32_t* plus, int32_t* minus
 extern "C" int pyqed_heom_tables_export(const pyqed_heom_plan* P, int32_t inst, int64_t* keys, int32_t* plus,
                                         int32_t* minus) {
   if (!P || inst < 0 || inst >= P->n_inst) return fail(PYQED_EINVAL, "tables_export: bad plan or instance");
-  export_tables(P->tabs.at({P->inst_K[inst], P->inst_L[inst]}), keys, plus, minus);
+  export_tables(*P->tabs.at({P->inst_K[inst], P->inst_L[inst]}), keys, plus, minus);
   return PYQED_OK;
 }
 
@@ -843,10 +1237,10 @@ extern "C" int pyqed_heom_tables_size(int32_t K, int32_t L, int64_t* nmax) {
 
 extern "C" int pyqed_heom_tables_build(int32_t K, int32_t L, int64_t* keys, int32_t* plus, int32_t* minus) {
   if (K < 0 || K > kMaxK || L < 0 || L > kMaxL) return fail(PYQED_EINVAL, "tables_build: bad K or L");
-  Tables t;
-  int rc = build_tables(K, L, t);
+  std::shared_ptr<const Tables> t;
+  int rc = cached_tables(K, L, t);
   if (rc) return rc;
-  export_tables(t, keys, plus, minus);
+  export_tables(*t, keys, plus, minus);
   return PYQED_OK;
 }
 
@@ -863,6 +1257,10 @@ StageParams base_params(const pyqed_heom_plan* P) {
   p.mode = P->d_mode;
   p.mat = P->d_mat;
   p.resident = P->d_res;
+  p.p_rowptr = P->d_prowptr;
+  p.p_ent = P->d_pent;
+  p.p_gs = P->d_pgs;
+  p.coefp = P->d_coefp;
   p.traj_stride = P->traj_stride;
   p.d2max = P->d2max;
   p.qmax = P->qmax;
@@ -885,6 +1283,35 @@ int stream_grid(const pyqed_heom_plan* P, int n_items) {
   int g = std::min(n_items, P->num_cu * 8);
   g = ((g + 7) / 8) * 8;
   return std::max(g, 8);
+}
+
+// One RK4 stage over every streamed item class: generic kernel, projector kernel with VALU
+// commutator, projector kernel with MFMA commutator (FP32 only; FP64 uses VALU).
+template <typename T, int S>
+int launch_stage(const pyqed_heom_plan* P, StageParams p, hipStream_t st) {
+  using TI = typename std::conditional<S == 0, double, T>::type;
+  for (int c = 0; c < 3; ++c) {
+    const int n = P->cls_begin[c + 1] - P->cls_begin[c];
+    if (n == 0) continue;
+    p.item_begin = P->cls_begin[c];
+    p.item_end = P->cls_begin[c + 1];
+    p.use_xcd = 1;
+    std::memcpy(p.xr, P->xr[c], sizeof(p.xr));
+    const int grid = stream_grid(P, n);
+    int rc;
+    if (c == 0) {
+      rc = launch(heom_stage_kernel<T, TI, T, S>, grid, stream_lds_bytes(P, sizeof(T)), st, p);
+    } else if (c == 2) {
+      if constexpr (std::is_same<T, float>::value)
+        rc = launch(heom_proj_kernel<T, TI, T, S, true>, grid, proj_lds_bytes(P, sizeof(T)), st, p);
+      else
+        rc = launch(heom_proj_kernel<T, TI, T, S, false>, grid, proj_lds_bytes(P, sizeof(T)), st, p);
+    } else {
+      rc = launch(heom_proj_kernel<T, TI, T, S, false>, grid, proj_lds_bytes(P, sizeof(T)), st, p);
+    }
+    if (rc) return rc;
+  }
+  return PYQED_OK;
 }
 
 template <typename T>
@@ -915,28 +1342,22 @@ int run_typed(pyqed_heom_plan* P, double2* y, char* ws, const double* field, int
     PQA_HIP(hipEventRecord(P->ev_join, P->side));
   }
   if (P->n_items_stream > 0) {
-    p.item_begin = 0;
-    p.item_end = P->n_items_stream;
-    p.use_xcd = 1;
-    std::memcpy(p.xr, P->xr, sizeof(p.xr));
-    const int grid = stream_grid(P, P->n_items_stream);
-    const size_t lds = stream_lds_bytes(P, sizeof(T));
     for (int64_t s = 0; s < nsteps; ++s) {
       p.step = step0 + s;
       int rc;
       p.xin = y;
       p.xout = x0;
-      if ((rc = launch(heom_stage_kernel<T, double, T, 0>, grid, lds, st, p))) return rc;
+      if ((rc = launch_stage<T, 0>(P, p, st))) return rc;
       p.xin = x0;
       p.xout = x1;
-      if ((rc = launch(heom_stage_kernel<T, T, T, 1>, grid, lds, st, p))) return rc;
+      if ((rc = launch_stage<T, 1>(P, p, st))) return rc;
       p.xin = x1;
       p.xout = x0;
-      if ((rc = launch(heom_stage_kernel<T, T, T, 2>, grid, lds, st, p))) return rc;
+      if ((rc = launch_stage<T, 2>(P, p, st))) return rc;
       p.xin = x0;
       p.xout = nullptr;
       p.save_slot = (traj && ((s + 1) % save_every) == 0) ? (s + 1) / save_every - 1 : -1;
-      if ((rc = launch(heom_stage_kernel<T, T, T, 3>, grid, lds, st, p))) return rc;
+      if ((rc = launch_stage<T, 3>(P, p, st))) return rc;
       p.save_slot = -1;
     }
   }

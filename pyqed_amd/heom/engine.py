@@ -79,7 +79,7 @@ def prepare_instance(inst):
 class HeomEngine:
     """A plan over one or more HEOM instances plus its device state."""
 
-    def __init__(self, instances, *, resident=True):
+    def __init__(self, instances, *, resident=True, generic=False):
         self.torch = torch = _lib.require_gpu()
         self.L = _lib.lib()
         self.prep = [prepare_instance(i) for i in instances]
@@ -105,7 +105,8 @@ class HeomEngine:
             s.d, s.K, s.L, s.ncoup = d, len(p["mode"]), p["L"], p["qdiag"].shape[0]
             for name, b in bufs.items():
                 setattr(s, name, None if b is None else b.ctypes.data)
-        flags = (0 if resident else _lib.PLAN_NO_RESIDENT) | (_lib.PLAN_DENSE_Q if self.dense else 0)
+        flags = ((0 if resident else _lib.PLAN_NO_RESIDENT) | (_lib.PLAN_DENSE_Q if self.dense else 0) |
+                 (_lib.PLAN_GENERIC if generic else 0))
         plan = C.c_void_p()
         _lib.check(self.L.pyqed_heom_plan_create(arr, self.n, flags, C.byref(plan)))
         self.plan = plan
@@ -144,6 +145,23 @@ class HeomEngine:
         if self._ws is None or self._ws.numel() < need:
             self._ws = torch.empty(max(need, 256), dtype=torch.uint8, device=self.device)
         return self._ws
+
+    def set_state(self, states):
+        """Load full hierarchy states (list of [nmax, d, d] arrays in the caller's basis) into y."""
+        torch = self.torch
+        for k, p in enumerate(self.prep):
+            a = np.asarray(states[k], np.complex128)
+            U = p["U"]
+            if U is not None:
+                a = U.conj().T @ a @ U
+            o = int(self.state_offset[k])
+            self.y[o:o + a.size] = torch.as_tensor(a.reshape(-1), device=self.device)
+
+    def classes(self):
+        it = np.zeros(3, np.int64)
+        el = np.zeros(3, np.int64)
+        _lib.check(self.L.pyqed_heom_plan_classes(self.plan, it.ctypes.data, el.ctypes.data))
+        return dict(items=it.tolist(), elems=el.tolist())
 
     def stats(self):
         s = np.zeros(4, np.int64)
