@@ -41,6 +41,10 @@ extern "C" {
 #define PYQED_PLAN_NO_RESIDENT 1 /* never run instances LDS-resident (always stream through HBM) */
 #define PYQED_PLAN_DENSE_Q 2     /* coupling operators given as dense d x d matrices (qmat) */
 #define PYQED_PLAN_GENERIC 4     /* do not use the site-projector kernels (for cross-checking) */
+#define PYQED_PLAN_HERMITIAN 8   /* the caller's initial states are Hermitian; with Hermitian H, real
+                                    couplings, real expn and etar = conj(etal) every ADO then stays
+                                    Hermitian, and the kernels read column terms as conjugated rows.
+                                    Rejected (PYQED_EINVAL) when the coefficients do not allow it. */
 
 /* Thread-local description of the last error (never NULL). */
 const char* pyqed_last_error(void);

@@ -8,7 +8,8 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpyqed_amd.so")
+# PYQED_AMD_LIB selects another build of the library (e.g. a variant compiled with other tuning).
+LIB_PATH = os.environ.get("PYQED_AMD_LIB") or os.path.join(_HERE, "libpyqed_amd.so")
 
 PYQED_OK = 0
 PREC_FP64 = 0
@@ -16,6 +17,7 @@ PREC_MIXED = 1
 PLAN_NO_RESIDENT = 1
 PLAN_DENSE_Q = 2
 PLAN_GENERIC = 4
+PLAN_HERMITIAN = 8
 
 SYMBOLS = [
     "pyqed_last_error", "pyqed_version", "pyqed_device_count",

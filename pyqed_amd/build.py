@@ -26,9 +26,9 @@ def hipcc():
     raise RuntimeError("hipcc not found (ROCm is required to build pyqed_amd)")
 
 
-def build(verbose=True):
+def build(verbose=True, out=None, defines=()):
     cc = hipcc()
-    flags = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", INCLUDE, "-I", CSRC,
+    flags = [*(f"-D{d}" for d in defines), "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", INCLUDE, "-I", CSRC,
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
     objs = []
     bdir = os.path.join(HERE, "_build")
@@ -41,15 +41,21 @@ def build(verbose=True):
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    target = out or LIB
+    tmp = target + ".tmp"
     cmd = [cc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, target)
+    return target
 
 
 if __name__ == "__main__":
-    build()
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=None, help="output path (default: pyqed_amd/libpyqed_amd.so)")
+    ap.add_argument("-D", dest="defines", action="append", default=[], help="preprocessor define NAME=VALUE")
+    a = ap.parse_args()
+    build(out=a.out, defines=a.defines)
     sys.exit(0)

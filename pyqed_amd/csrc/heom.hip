@@ -59,15 +59,15 @@ struct InstDev {
   int32_t h_off, q_off;  // offsets (complex elements) into the matrix pool
   // site-projector path: site-ordered tables and coefficients scaled by the projector weight
   int32_t path;          // 0 generic, 1 projector/VALU commutator, 2 projector/MFMA commutator
-  int32_t p_rowptr_off, p_ent_off, coefp_off;
-  int64_t p_gs_off;      // byte offset of this instance's site-group starts [nmax][d+1]
+  int32_t coefp_off;     // coefficients scaled by the projector weights
+  int32_t site_off;      // site of every exponential, [K]
 };
 
 struct WorkItem {
   int32_t inst, ado0, nado, pad;
 };
 
-enum { F_SDIP = 1, F_CDIP = 2, F_DENSE = 4 };
+enum { F_SDIP = 1, F_CDIP = 2, F_DENSE = 4, F_HERM = 8 };
 
 struct StageParams {
   const InstDev* insts;
@@ -78,9 +78,7 @@ struct StageParams {
   const int32_t* mode;
   const double2* mat;   // H, sdip, qdiag/qmat, cdiag/cmat
   const int32_t* resident;  // resident instance ids (resident kernel)
-  const int32_t* p_rowptr;  // site-ordered tables (projector path)
-  const int2* p_ent;
-  const uint8_t* p_gs;
+  const int32_t* site;      // projector path: site of each exponential
   const double2* coefp;
   const double* field;
   int64_t step;
@@ -98,6 +96,7 @@ struct StageParams {
   int32_t xr[9];  // XCD ranges of the item queue (item_begin-relative), xr[8] = count
   int32_t use_xcd;
   int32_t col_override;  // >= 0: field column used instead of the stage's own (RHS-only evaluation)
+  int32_t rowbuf, colbuf;  // projector path: per-wave neighbour row / column buffer (complex elements)
   int32_t d2max, qmax, Kmax, Lmax;
 };
 
@@ -122,10 +121,10 @@ __device__ __forceinline__ Lds<T> carve(char* base, const StageParams& p) {
   o += sizeof(cpx<T>) * p.qmax;
   l.C = (cpx<T>*)(base + o);
   o += sizeof(cpx<T>) * 5 * p.Kmax;
+  l.S = (T*)(base + o);  // 8-byte data before the int array keeps every field aligned
+  o += sizeof(T) * (p.Lmax + 2);
   l.M = (int32_t*)(base + o);
   o += sizeof(int32_t) * p.Kmax;
-  l.S = (T*)(base + o);
-  o += sizeof(T) * (p.Lmax + 2);
   o = (o + 15) & ~size_t(15);
   l.rest = base + o;
   return l;
@@ -268,6 +267,39 @@ __device__ __forceinline__ cpx<T> element_rhs(const StageParams& p, const InstDe
   return r;
 }
 
+// Values an element's RK4 update reads, prefetched ahead of the right-hand side.
+struct Pre {
+  double2 y, an;
+};
+
+template <int STAGE>
+__device__ __forceinline__ Pre rk4_prefetch(const StageParams& p, int64_t g) {
+  Pre v;
+  v.y = (STAGE <= 2) ? p.y[g] : make_double2(0, 0);
+  v.an = (STAGE >= 1 && STAGE <= 3) ? p.ynew[g] : make_double2(0, 0);
+  return v;
+}
+
+template <typename TO, int STAGE>
+__device__ __forceinline__ void rk4_store(const StageParams& p, int64_t g, double rx, double ry, const Pre& v,
+                                          cpx_store<TO>* xout, bool save, int64_t tpos) {
+  const double dt = p.dt;
+  if (STAGE == 4) {
+    ((double2*)p.xout)[g] = make_double2(rx, ry);
+  } else if (STAGE == 0) {
+    p.ynew[g] = make_double2(v.y.x + rx * (dt / 6), v.y.y + ry * (dt / 6));
+    st(xout + g, mk<TO>((TO)(v.y.x + rx * (dt / 2)), (TO)(v.y.y + ry * (dt / 2))));
+  } else if (STAGE == 1 || STAGE == 2) {
+    p.ynew[g] = make_double2(v.an.x + rx * (dt / 3), v.an.y + ry * (dt / 3));
+    const double h = STAGE == 1 ? dt / 2 : dt;
+    st(xout + g, mk<TO>((TO)(v.y.x + rx * h), (TO)(v.y.y + ry * h)));
+  } else {
+    const double2 yn = make_double2(v.an.x + rx * (dt / 6), v.an.y + ry * (dt / 6));
+    p.y[g] = yn;
+    if (save) p.traj[tpos] = yn;
+  }
+}
+
 // RK4 stage update of element g (absolute index) from its right-hand side (rx, ry).
 // `save` marks ADO 0 of an instance on a trajectory step; tpos is its slot index.
 template <typename TO, int STAGE>
@@ -357,16 +389,24 @@ __global__ __launch_bounds__(kThreads) void heom_stage_kernel(StageParams p) {
 }
 
 // ------------------------------------------------------------------ site-projector stage kernel
-// For couplings Q_m = c_m |p_m><p_m| the coupling term of element (i, j) of ADO n is
-//   sum_{k: site(k) = i} c cL_k X'_k(i, j) + sum_{k: site(k) = j} c cR_k X'_k(i, j),
-// i.e. only the neighbours coupled through site i or site j contribute, each at the element's own
-// position.  The tables list every ADO's neighbours grouped by site (gs[a][s] = first entry of site
-// s), so each element visits only its two groups.  One wavefront owns one ADO: a prologue stages
-// the ADO's neighbour list with its coefficients in LDS and reduces the damping term; the
-// commutator runs on VALU (d <= 11, d >= 17) or on FP32 MFMA 16x16x4 (12 <= d <= 16, padded to 16).
+// For couplings Q_m = c_m |p_m><p_m| the coupling term of a neighbour X' entering through
+// exponential k (site s = p_{mode[k]}) touches only row s and column s of the output:
+//   out(s, j) += c cL_k X'(s, j),   out(i, s) += c cR_k X'(i, s).
+// One wavefront owns one ADO and reaches memory in two round trips:
+//   1. its neighbour list, its own matrix and the RK4 operands (y, ynew) it will update;
+//   2. row s of every neighbour, lanes spread over (entry, column), staged in an LDS row buffer
+//      (for non-Hermitian ADOs column s as well).
+// The neighbour list is sorted by site in LDS (ballot counting sort), so element (i, j) reads only
+// its two groups from the row buffer: sum_{site(t)=i} cL_t R_t(j) + sum_{site(t)=j} cR_t R_t(i)^*
+// (Hermitian ADOs: X'(i, s) = conj(X'(s, i))).  The commutator -i[H, X] runs on FP32 MFMA 16x16x4
+// for 12 <= d <= 16 and on VALU otherwise, while the round-trip-2 loads are in flight.
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxEnt = 64;  // neighbour entries per ADO handled by the projector path
+#ifndef PQA_PROJ_WAVES
+#define PQA_PROJ_WAVES 3  // minimum waves per SIMD the projector kernel is compiled for
+#endif
+constexpr int kChunk = 8;    // neighbour loads per lane issued together
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -387,37 +427,44 @@ __device__ __forceinline__ cpx<T> wave_sum(cpx<T> v) {
 template <typename T>
 struct WaveLds {
   cpx<T>* X;    // own ADO, d*d
-  int32_t* nb;  // neighbour ADO index per entry (site-ordered)
-  cpx<T>* cl;   // coefficient of the row term
-  cpx<T>* cr;   // coefficient of the column term
-  int32_t* gs;  // d+1 site-group starts
+  cpx<T>* R;    // neighbour rows R[t][0..d), t in site order
+  cpx<T>* Cc;   // neighbour columns (non-Hermitian only)
+  cpx<T>* cl;   // per entry: coefficient of the row term
+  cpx<T>* cr;   // per entry: coefficient of the column term
+  int32_t* nb;  // per entry: neighbour ADO
+  int32_t* st;  // per entry: site
+  int32_t* gs;  // site-group starts, d+1
 };
 
-__host__ __device__ inline size_t proj_wave_bytes(int d2max, size_t ts) {
-  size_t b = 2 * ts * (size_t)d2max + 4 * kMaxEnt + 2 * 2 * ts * kMaxEnt + 4 * 40;
+__host__ __device__ inline size_t proj_wave_bytes(int d2max, int rowbuf, int colbuf, size_t ts) {
+  size_t b = 2 * ts * ((size_t)d2max + rowbuf + colbuf) + 2 * 2 * ts * kMaxEnt + 2 * 4 * kMaxEnt + 4 * 36;
   return (b + 15) & ~size_t(15);
 }
 __host__ __device__ inline size_t proj_block_bytes(int d2max, int Kmax, int Lmax, size_t ts) {
-  size_t b = 2 * ts * (size_t)d2max + 2 * ts * 5 * (size_t)Kmax + ts * (size_t)(Lmax + 2);
+  size_t b = 2 * ts * (size_t)d2max + 2 * ts * 5 * (size_t)Kmax + ts * (size_t)(Lmax + 2) + 4 * (size_t)Kmax;
   return (b + 15) & ~size_t(15);
 }
 
-template <typename T, typename TI, typename TO, int STAGE, bool MFMA>
-__global__ __launch_bounds__(kThreads) void heom_proj_kernel(StageParams p) {
+template <typename T, typename TI, typename TO, int STAGE, bool MFMA, bool HERM>
+__global__ __launch_bounds__(kThreads, PQA_PROJ_WAVES) void heom_proj_kernel(StageParams p) {
   extern __shared__ __align__(16) char smem[];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   cpx<T>* sH = (cpx<T>*)smem;
   cpx<T>* sC = sH + p.d2max;
   T* sS = (T*)(sC + 5 * p.Kmax);
+  int32_t* sSite = (int32_t*)(sS + p.Lmax + 2);
   char* wbase = smem + proj_block_bytes(p.d2max, p.Kmax, p.Lmax, sizeof(T));
   WaveLds<T> w;
   {
-    char* b = wbase + wv * proj_wave_bytes(p.d2max, sizeof(T));
+    char* b = wbase + wv * proj_wave_bytes(p.d2max, p.rowbuf, p.colbuf, sizeof(T));
     w.X = (cpx<T>*)b;
-    w.cl = w.X + p.d2max;
+    w.R = w.X + p.d2max;
+    w.Cc = w.R + p.rowbuf;
+    w.cl = w.Cc + p.colbuf;
     w.cr = w.cl + kMaxEnt;
     w.nb = (int32_t*)(w.cr + kMaxEnt);
-    w.gs = w.nb + kMaxEnt;
+    w.st = w.nb + kMaxEnt;
+    w.gs = w.st + kMaxEnt;
   }
   const cpx_store<TI>* xin = (const cpx_store<TI>*)p.xin;
   cpx_store<TO>* xout = (cpx_store<TO>*)p.xout;
@@ -428,6 +475,8 @@ __global__ __launch_bounds__(kThreads) void heom_proj_kernel(StageParams p) {
     const WorkItem wi = p.items[it];
     const InstDev I = p.insts[wi.inst];
     const int d = I.d, d2 = I.d2;
+    // floor(idx / d) = (idx * dmag) >> 20 for idx < 2^11, d <= 32 (dmag*d - 2^20 < d < 2^20 / 2^11)
+    const uint32_t dmag = ((1u << 20) + d - 1) / d;
     __syncthreads();
     {
       double fs = 0.0;
@@ -447,95 +496,163 @@ __global__ __launch_bounds__(kThreads) void heom_proj_kernel(StageParams p) {
         const double2 v = p.coefp[I.coefp_off + e];
         sC[e] = mk<T>((T)v.x, (T)v.y);
       }
+      for (int e = tid; e < I.K; e += kThreads) sSite[e] = p.site[I.site_off + e];
       for (int e = tid; e < I.L + 2; e += kThreads) sS[e] = (T)sqrt((double)e);
     }
     __syncthreads();
-    // H fragments for the MFMA commutator (lane = (gk, gi), gk = lane / 16, gi = lane % 16)
+    // MFMA lane coordinates: lane = (gk, gi), gk = lane / 16, gi = lane % 16
     const int gi = lane & 15, gk = lane >> 4;
-    float hAr[4], hAi[4], nhAi[4], hBi[4], nhBr[4], nhBi[4];
-    if (MFMA) {
-#pragma unroll
-      for (int kc = 0; kc < 4; ++kc) {
-        const int k = 4 * kc + gk;
-        const bool ok = gi < d && k < d;
-        const cpx<T> a = ok ? sH[gi * d + k] : mk<T>(0, 0);  // A-layout H[gi][k]
-        const cpx<T> b = ok ? sH[k * d + gi] : mk<T>(0, 0);  // B-layout H[k][gi]
-        hAr[kc] = (float)a.x;
-        hAi[kc] = (float)a.y;
-        nhAi[kc] = -(float)a.y;
-        hBi[kc] = (float)b.y;
-        nhBr[kc] = -(float)b.x;
-        nhBi[kc] = -(float)b.y;
-      }
-    }
     const cpx_store<TI>* xi = xin + I.off;
-    for (int a = wi.ado0 + wv; a < wi.ado0 + wi.nado; a += kThreads / 64) {
+    const int rs = d + 1;  // row-buffer stride: odd, so rows do not start on the same LDS bank
+    // Software pipeline over this wave's ADOs: the neighbour list (and, on the MFMA path, the own
+    // matrix) of the next ADO are loaded while the current one is processed; they are issued after
+    // the current ADO's neighbour loads, so waiting for those does not wait for the prefetch.
+    const int aend = wi.ado0 + wi.nado;
+    int nent_n = 0;
+    int2 en_n = make_int2(0, 0);
+    cpx_store<TI> xt_n[4];
+    auto prefetch = [&](int an) {
+      const int r0 = p.rowptr[I.rowptr_off + an];
+      nent_n = p.rowptr[I.rowptr_off + an + 1] - r0;
+      en_n = make_int2(0, 0);
+      if (lane < nent_n) en_n = p.ent[I.ent_off + r0 + lane];
+      if (MFMA) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int e = lane + 64 * q;
+          if (e < d2) xt_n[q] = xi[(int64_t)an * d2 + e];
+        }
+      }
+    };
+    if (wi.ado0 + wv < aend) prefetch(wi.ado0 + wv);
+    for (int a = wi.ado0 + wv; a < aend; a += kThreads / 64) {
       const int64_t base = (int64_t)a * d2;
-      // ---- prologue: neighbour list with coefficients, damping, site groups, own matrix
-      const int rp0 = p.p_rowptr[I.p_rowptr_off + a];
-      const int nent = p.p_rowptr[I.p_rowptr_off + a + 1] - rp0;
+      // ---- neighbour list and own matrix (prefetched)
+      const int nent = nent_n;
+      const int2 en = en_n;
+      if (MFMA) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int e = lane + 64 * q;
+          if (e < d2) w.X[e] = cvt<T>(mk(xt_n[q].x, xt_n[q].y));
+        }
+      } else {
+        for (int e = lane; e < d2; e += 64) w.X[e] = ldc<T>(xi, base + e);
+      }
+      // ---- neighbour list sorted by site (counting sort over ballots) with coefficients
+      const int k = en.y & 0xffff, nf = (en.y >> 16) & 0x7f;
+      const bool plus = en.y < 0;
+      const int my_site = lane < nent ? sSite[k] : 0x7fffffff;
+      int pos = 0, gsv = 0, run = 0;
+      const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+      for (int s2 = 0; s2 < d; ++s2) {
+        const uint64_t m = __ballot(my_site == s2);
+        if (lane == s2) gsv = run;
+        if (my_site == s2) pos = run + __popcll(m & below);
+        run += __popcll(m);
+      }
+      if (lane == d) gsv = run;
+      if (lane <= d) w.gs[lane] = gsv;
       cpx<T> dp = mk<T>(0, 0);
       if (lane < nent) {
-        const int2 en = p.p_ent[I.p_ent_off + rp0 + lane];
-        const int k = en.y & 0xffff, nf = (en.y >> 16) & 0x7f;
-        const bool plus = en.y < 0;
         const cpx<T>* C = sC + 5 * k;
         const T sq = sS[nf];
-        w.nb[lane] = en.x;
-        w.cl[lane] = (plus ? C[3] : C[1]) * sq;
-        w.cr[lane] = (plus ? C[4] : C[2]) * sq;
+        w.nb[pos] = en.x;
+        w.st[pos] = my_site;
+        w.cl[pos] = (plus ? C[3] : C[1]) * sq;
+        w.cr[pos] = (plus ? C[4] : C[2]) * sq;
         if (!plus) dp = C[0] * (T)nf;
       }
-      if (lane <= d) w.gs[lane] = p.p_gs[I.p_gs_off + (int64_t)a * (d + 1) + lane];
-      for (int e = lane; e < d2; e += 64) w.X[e] = ldc<T>(xi, base + e);
       const cpx<T> damp = wave_sum(dp);
       wave_sync();
-      const bool save_ado = STAGE == 3 && a == 0 && p.traj && p.save_slot >= 0;
-      const int64_t tbase = p.save_slot * p.traj_stride + I.traj_off;
+      // ---- round trip 2: row s (pass 0) and, unless Hermitian, column s (pass 1) of every
+      // neighbour into LDS; value idx = t*d + jj of a pass is element jj of entry t's row/column
+      const int nval = nent * d;
+      cpx<T> v[kChunk];
+      auto issue = [&](int pass, int q0) {
+#pragma unroll
+        for (int q = 0; q < kChunk; ++q) {
+          const int idx = q0 + lane + 64 * q;
+          if (idx < nval) {
+            const int t = (int)(((uint32_t)idx * dmag) >> 20), jj = idx - t * d, s2 = w.st[t];
+            v[q] = ldc<T>(xi, (int64_t)w.nb[t] * d2 + (pass == 0 ? s2 * d + jj : jj * d + s2));
+          }
+        }
+      };
+      auto stash = [&](int pass, int q0) {
+        cpx<T>* dst = pass == 0 ? w.R : w.Cc;
+#pragma unroll
+        for (int q = 0; q < kChunk; ++q) {
+          const int idx = q0 + lane + 64 * q;
+          if (idx < nval) {
+            const int t = (int)(((uint32_t)idx * dmag) >> 20);
+            dst[idx + t] = v[q];  // row t starts at t * (d + 1)
+          }
+        }
+      };
+      issue(0, 0);
+      if (a + kThreads / 64 < aend) prefetch(a + kThreads / 64);
+      // ---- commutator and damping (while the neighbour loads are in flight)
+      cpx<T> acc[4];
       if (MFMA) {
         floatx4 aR = {0.f, 0.f, 0.f, 0.f}, aI = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kc = 0; kc < 4; ++kc) {
-          const int k = 4 * kc + gk;
-          const bool ok = gi < d && k < d;
-          const cpx<T> xb = ok ? w.X[k * d + gi] : mk<T>(0, 0);  // B-layout X[k][gi]
-          const cpx<T> xa = ok ? w.X[gi * d + k] : mk<T>(0, 0);  // A-layout X[gi][k]
+          const int kk = 4 * kc + gk;
+          const bool ok = gi < d && kk < d;
+          const cpx<T> xb = ok ? w.X[kk * d + gi] : mk<T>(0, 0);  // B-layout X[kk][gi]
+          const cpx<T> xa = ok ? w.X[gi * d + kk] : mk<T>(0, 0);  // A-layout X[gi][kk]
+          const cpx<T> ha = ok ? sH[gi * d + kk] : mk<T>(0, 0);   // A-layout H[gi][kk]
+          const cpx<T> hb = ok ? sH[kk * d + gi] : mk<T>(0, 0);   // B-layout H[kk][gi]
           const float xbr = (float)xb.x, xbi = (float)xb.y, xar = (float)xa.x, xai = (float)xa.y;
+          const float har = (float)ha.x, hai = (float)ha.y, hbr = (float)hb.x, hbi = (float)hb.y;
           // + H X
-          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(hAr[kc], xbr, aR, 0, 0, 0);
-          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(nhAi[kc], xbi, aR, 0, 0, 0);
-          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(hAr[kc], xbi, aI, 0, 0, 0);
-          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(hAi[kc], xbr, aI, 0, 0, 0);
+          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(har, xbr, aR, 0, 0, 0);
+          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(-hai, xbi, aR, 0, 0, 0);
+          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(har, xbi, aI, 0, 0, 0);
+          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(hai, xbr, aI, 0, 0, 0);
           // - X H
-          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(xar, nhBr[kc], aR, 0, 0, 0);
-          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(xai, hBi[kc], aR, 0, 0, 0);
-          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(xar, nhBi[kc], aI, 0, 0, 0);
-          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(xai, nhBr[kc], aI, 0, 0, 0);
+          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(xar, -hbr, aR, 0, 0, 0);
+          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(xai, hbi, aR, 0, 0, 0);
+          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(xar, -hbi, aI, 0, 0, 0);
+          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(xai, -hbr, aI, 0, 0, 0);
         }
-        // elements (4 gk + r, gi), r = 0..3
-        cpx<T> acc[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int i = 4 * gk + r;
           acc[r] = mul_mi(mk<T>((T)aR[r], (T)aI[r]));
           if (i < d && gi < d) cmsub(acc[r], damp, w.X[i * d + gi]);
         }
+      }
+      stash(0, 0);
+      for (int pass = 0; pass < (HERM ? 1 : 2); ++pass)
+        for (int q0 = pass == 0 ? 64 * kChunk : 0; q0 < nval; q0 += 64 * kChunk) {
+          issue(pass, q0);
+          stash(pass, q0);
+        }
+      wave_sync();
+      // ---- gather the neighbour terms from the row buffer, RK4 update
+      const bool save_ado = STAGE == 3 && a == 0 && p.traj && p.save_slot >= 0;
+      const int64_t tbase = p.save_slot * p.traj_stride + I.traj_off;
+      if (MFMA) {
         if (gi < d) {
+          // column term, site gi: element (4gk+r, gi) <- cR_t * conj(R_t[4gk+r])
           for (int t = w.gs[gi]; t < w.gs[gi + 1]; ++t) {
-            const int64_t nb = (int64_t)w.nb[t] * d2 + gi;
             const cpx<T> c = w.cr[t];
+            const cpx<T>* row = (HERM ? w.R : w.Cc) + t * rs + 4 * gk;
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-              if (4 * gk + r < d) cmac(acc[r], c, ldc<T>(xi, nb + (4 * gk + r) * d));
+              if (4 * gk + r < d) cmac(acc[r], c, HERM ? conjg(row[r]) : row[r]);
           }
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int i = 4 * gk + r;
             if (i < d) {
-              for (int t = w.gs[i]; t < w.gs[i + 1]; ++t)
-                cmac(acc[r], w.cl[t], ldc<T>(xi, (int64_t)w.nb[t] * d2 + i * d + gi));
+              // row term, site i: element (i, gi) <- cL_t * R_t[gi]
+              for (int t = w.gs[i]; t < w.gs[i + 1]; ++t) cmac(acc[r], w.cl[t], w.R[t * rs + gi]);
               const int e = i * d + gi;
-              rk4_apply<TO, STAGE>(p, I.off + base + e, (double)acc[r].x, (double)acc[r].y, xout, save_ado,
+              const Pre pr = rk4_prefetch<STAGE>(p, I.off + base + e);
+              rk4_store<TO, STAGE>(p, I.off + base + e, (double)acc[r].x, (double)acc[r].y, pr, xout, save_ado,
                                    tbase + e);
             }
           }
@@ -543,16 +660,18 @@ __global__ __launch_bounds__(kThreads) void heom_proj_kernel(StageParams p) {
       } else {
         for (int e = lane; e < d2; e += 64) {
           const int i = e / d, j = e - (e / d) * d;
+          const Pre pr = rk4_prefetch<STAGE>(p, I.off + base + e);
           cpx<T> c = mk<T>(0, 0);
-          for (int k = 0; k < d; ++k) {
-            cmac(c, sH[i * d + k], w.X[k * d + j]);
-            cmsub(c, w.X[i * d + k], sH[k * d + j]);
+          for (int kk = 0; kk < d; ++kk) {
+            cmac(c, sH[i * d + kk], w.X[kk * d + j]);
+            cmsub(c, w.X[i * d + kk], sH[kk * d + j]);
           }
-          cpx<T> r = mul_mi(c);
-          cmsub(r, damp, w.X[e]);
-          for (int t = w.gs[i]; t < w.gs[i + 1]; ++t) cmac(r, w.cl[t], ldc<T>(xi, (int64_t)w.nb[t] * d2 + e));
-          for (int t = w.gs[j]; t < w.gs[j + 1]; ++t) cmac(r, w.cr[t], ldc<T>(xi, (int64_t)w.nb[t] * d2 + e));
-          rk4_apply<TO, STAGE>(p, I.off + base + e, (double)r.x, (double)r.y, xout, save_ado, tbase + e);
+          cpx<T> o = mul_mi(c);
+          cmsub(o, damp, w.X[e]);
+          for (int t = w.gs[i]; t < w.gs[i + 1]; ++t) cmac(o, w.cl[t], w.R[t * rs + j]);
+          for (int t = w.gs[j]; t < w.gs[j + 1]; ++t)
+            cmac(o, w.cr[t], HERM ? conjg(w.R[t * rs + i]) : w.Cc[t * rs + i]);
+          rk4_store<TO, STAGE>(p, I.off + base + e, (double)o.x, (double)o.y, pr, xout, save_ado, tbase + e);
         }
       }
       wave_sync();
@@ -749,49 +868,6 @@ int cached_tables(int K, int L, std::shared_ptr<const Tables>& out) {
   return PYQED_OK;
 }
 
-// Site-ordered copy of a (K, L) table for the projector path: every ADO's entries sorted (stably)
-// by the site their exponential couples to, plus gs[a][s] = number of the ADO's entries with
-// site < s (s = 0..d).
-struct ProjTables {
-  std::vector<int32_t> rowptr;
-  std::vector<int2> ent;
-  std::vector<uint8_t> gs;  // [nmax][d+1]
-  int max_ent = 0;
-};
-
-int cached_proj_tables(const Tables& t, int d, const std::vector<int>& site, std::shared_ptr<const ProjTables>& out) {
-  static std::mutex mu;
-  static std::map<std::tuple<int, int, int, std::vector<int>>, std::shared_ptr<const ProjTables>> cache;
-  const auto key = std::make_tuple(t.K, t.L, d, site);
-  {
-    std::lock_guard<std::mutex> g(mu);
-    auto it = cache.find(key);
-    if (it != cache.end()) {
-      out = it->second;
-      return PYQED_OK;
-    }
-  }
-  auto pt = std::make_shared<ProjTables>();
-  pt->rowptr = t.rowptr;
-  pt->ent.resize(t.ent.size());
-  pt->gs.assign((size_t)t.nmax * (d + 1), 0);
-  std::vector<int> cnt(d + 1);
-  for (int a = 0; a < t.nmax; ++a) {
-    const int r0 = t.rowptr[a], r1 = t.rowptr[a + 1];
-    pt->max_ent = std::max(pt->max_ent, r1 - r0);
-    std::fill(cnt.begin(), cnt.end(), 0);
-    for (int q = r0; q < r1; ++q) cnt[site[t.ent[q].y & 0xffff] + 1]++;
-    for (int s2 = 0; s2 < d; ++s2) cnt[s2 + 1] += cnt[s2];
-    for (int s2 = 0; s2 <= d; ++s2) pt->gs[(size_t)a * (d + 1) + s2] = (uint8_t)std::min(cnt[s2], 255);
-    for (int q = r0; q < r1; ++q) pt->ent[r0 + cnt[site[t.ent[q].y & 0xffff]]++] = t.ent[q];
-  }
-  std::lock_guard<std::mutex> g(mu);
-  auto& slot = cache[key];
-  if (!slot) slot = pt;
-  out = slot;
-  return PYQED_OK;
-}
-
 template <typename K>
 void* kernel_ptr(K k) {
   return reinterpret_cast<void*>(k);
@@ -804,7 +880,6 @@ struct pyqed_heom_plan {
   std::vector<InstDev> inst;
   std::vector<int64_t> nmax;
   std::map<std::pair<int, int>, std::shared_ptr<const Tables>> tabs;
-  std::vector<std::shared_ptr<const ProjTables>> ptabs;
   std::vector<int> inst_K, inst_L;
   int64_t n_elems = 0, n_ados = 0, traj_stride = 0;
   int d2max = 1, qmax = 1, Kmax = 1, Lmax = 0;
@@ -824,11 +899,11 @@ struct pyqed_heom_plan {
   int32_t* d_mode = nullptr;
   double2* d_mat = nullptr;
   int32_t* d_res = nullptr;
-  int32_t* d_prowptr = nullptr;
-  int2* d_pent = nullptr;
-  uint8_t* d_pgs = nullptr;
+  int32_t* d_site = nullptr;
   double2* d_coefp = nullptr;
   int n_proj = 0;
+  int rowbuf = 1, colbuf = 0;  // projector path: max entries*d over instances (columns: non-Hermitian)
+  bool hermitian = false;
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
@@ -853,7 +928,8 @@ size_t stream_lds_bytes(const pyqed_heom_plan* P, size_t tsize) {
 }
 
 size_t proj_lds_bytes(const pyqed_heom_plan* P, size_t tsize) {
-  return proj_block_bytes(P->d2max, P->Kmax, P->Lmax, tsize) + (kThreads / 64) * proj_wave_bytes(P->d2max, tsize);
+  return proj_block_bytes(P->d2max, P->Kmax, P->Lmax, tsize) +
+         (kThreads / 64) * proj_wave_bytes(P->d2max, P->rowbuf, P->colbuf, tsize);
 }
 
 size_t resident_lds_bytes(const pyqed_heom_plan* P, int64_t n_el, size_t tsize) {
@@ -877,9 +953,7 @@ void destroy_plan(pyqed_heom_plan* P) {
   (void)hipFree(P->d_mode);
   (void)hipFree(P->d_mat);
   (void)hipFree(P->d_res);
-  (void)hipFree(P->d_prowptr);
-  (void)hipFree(P->d_pent);
-  (void)hipFree(P->d_pgs);
+  (void)hipFree(P->d_site);
   (void)hipFree(P->d_coefp);
   if (P->side) (void)hipStreamDestroy(P->side);
   if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
@@ -898,6 +972,7 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
   pyqed_heom_plan* P = new pyqed_heom_plan();
   P->n_inst = n_inst;
   P->flags = flags;
+  P->hermitian = (flags & PYQED_PLAN_HERMITIAN) != 0;
   const bool dense = (flags & PYQED_PLAN_DENSE_Q) != 0;
   // ---- validate and build tables
   for (int i = 0; i < n_inst; ++i) {
@@ -948,10 +1023,7 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     ent.insert(ent.end(), kv.second->ent.begin(), kv.second->ent.end());
   }
   std::vector<double2> coef, mat, coefp;
-  std::vector<int32_t> mode, prowptr;
-  std::vector<int2> pent;
-  std::vector<uint8_t> pgs;
-  std::map<const ProjTables*, std::tuple<int, int, int64_t>> ptab_off;
+  std::vector<int32_t> mode, sites;
   int64_t off = 0, toff = 0, ados = 0;
   for (int i = 0; i < n_inst; ++i) {
     const pyqed_heom_instance& s = in[i];
@@ -991,6 +1063,26 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     for (int e = 0; e < nq; ++e) mat.push_back(make_double2(qsrc[2 * e], qsrc[2 * e + 1]));
     if (csrc)
       for (int e = 0; e < nq; ++e) mat.push_back(make_double2(csrc[2 * e], csrc[2 * e + 1]));
+    if (flags & PYQED_PLAN_HERMITIAN) {
+      // the caller asserts Hermitian initial states; check what makes the dynamics preserve it
+      bool ok = !dense && !s.cdiag;
+      for (int a2 = 0; a2 < s.d && ok; ++a2)
+        for (int b2 = 0; b2 < s.d && ok; ++b2) {
+          ok = std::abs(cz(s.H, a2 * s.d + b2) - std::conj(cz(s.H, b2 * s.d + a2))) <= 1e-14 * (1 + std::abs(cz(s.H, a2 * s.d + b2)));
+          if (ok && s.sdip)
+            ok = std::abs(cz(s.sdip, a2 * s.d + b2) - std::conj(cz(s.sdip, b2 * s.d + a2))) <= 1e-14 * (1 + std::abs(cz(s.sdip, a2 * s.d + b2)));
+        }
+      for (int e = 0; e < s.ncoup * s.d && ok; ++e) ok = cz(s.qdiag, e).imag() == 0.0;
+      for (int k = 0; k < s.K && ok; ++k)
+        ok = cz(s.expn, k).imag() == 0.0 && std::abs(cz(s.etar, k) - std::conj(cz(s.etal, k))) <= 1e-14 * std::abs(cz(s.etal, k));
+      if (!ok) {
+        destroy_plan(P);
+        return fail(PYQED_EINVAL, "instance " + std::to_string(i) +
+                                      ": PYQED_PLAN_HERMITIAN needs Hermitian H and dipole, real diagonal couplings, "
+                                      "real expn and etar = conj(etal)");
+      }
+      I.flags |= F_HERM;
+    }
     // ---- site-projector path: every coupling is c_m |p_m><p_m| (diagonal with one nonzero entry)
     I.path = 0;
     if (!dense && !s.cdiag && s.d >= 6 && !(flags & PYQED_PLAN_GENERIC)) {
@@ -1009,43 +1101,28 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
         }
         proj = nz == 1;
       }
-      if (proj) {
-        std::vector<int> site(s.K);
-        for (int k = 0; k < s.K; ++k) site[k] = psite[s.mode[k]];
-        std::shared_ptr<const ProjTables> pt;
-        int rc = cached_proj_tables(t, s.d, site, pt);
-        if (rc) {
-          destroy_plan(P);
-          return rc;
-        }
-        if (pt->max_ent <= kMaxEnt) {
-          P->ptabs.push_back(pt);
-          I.path = (s.d >= 12 && s.d <= 16) ? 2 : 1;
-          if (!ptab_off.count(pt.get())) {
-            ptab_off[pt.get()] = std::make_tuple((int)prowptr.size(), (int)pent.size(), (int64_t)pgs.size());
-            prowptr.insert(prowptr.end(), pt->rowptr.begin(), pt->rowptr.end());
-            pent.insert(pent.end(), pt->ent.begin(), pt->ent.end());
-            pgs.insert(pgs.end(), pt->gs.begin(), pt->gs.end());
+      // projector path: at most kMaxEnt neighbour entries per ADO (K plus + min(K, L) minus)
+      if (proj && s.K + std::min(s.K, s.L) <= kMaxEnt) {
+        I.path = (s.d >= 12 && s.d <= 16) ? 2 : 1;
+        I.site_off = (int)sites.size();
+        for (int k = 0; k < s.K; ++k) sites.push_back(psite[s.mode[k]]);
+        I.coefp_off = (int)coefp.size();
+        for (int k = 0; k < s.K; ++k) {
+          const std::complex<double> wgt = pw[s.mode[k]];
+          for (int q = 0; q < 5; ++q) {
+            const double2 c = coef[I.coef_off + 5 * k + q];
+            std::complex<double> v(c.x, c.y);
+            if (q > 0) v *= wgt;
+            coefp.push_back(make_double2(v.real(), v.imag()));
           }
-          const auto& o = ptab_off[pt.get()];
-          I.p_rowptr_off = std::get<0>(o);
-          I.p_ent_off = std::get<1>(o);
-          I.p_gs_off = std::get<2>(o);
-          I.coefp_off = (int)coefp.size();
-          for (int k = 0; k < s.K; ++k) {
-            const std::complex<double> wgt = pw[s.mode[k]];
-            for (int q = 0; q < 5; ++q) {
-              const double2 c = coef[I.coef_off + 5 * k + q];
-              std::complex<double> v(c.x, c.y);
-              if (q > 0) v *= wgt;
-              coefp.push_back(make_double2(v.real(), v.imag()));
-            }
-          }
-          P->n_proj++;
         }
+        P->n_proj++;
+        const int nb = (s.K + std::min(s.K, s.L)) * (s.d + 1);
+        P->rowbuf = std::max(P->rowbuf, nb);
+        if (!(flags & PYQED_PLAN_HERMITIAN)) P->colbuf = std::max(P->colbuf, nb);
       }
     }
-    if (mat.size() > (size_t)INT32_MAX || coef.size() > (size_t)INT32_MAX || pent.size() > (size_t)INT32_MAX) {
+    if (mat.size() > (size_t)INT32_MAX || coef.size() > (size_t)INT32_MAX || coefp.size() > (size_t)INT32_MAX) {
       destroy_plan(P);
       return fail(PYQED_EINVAL, "coefficient pools exceed 2^31 entries");
     }
@@ -1125,8 +1202,7 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     }
     P->xr[c][8] = n;
   }
-  if (stream_lds_bytes(P, sizeof(double)) > (size_t)P->max_lds ||
-      proj_lds_bytes(P, sizeof(double)) > (size_t)P->max_lds) {
+  if (stream_lds_bytes(P, sizeof(double)) > (size_t)P->max_lds) {
     destroy_plan(P);
     return fail(PYQED_EUNSUPPORTED, "instance too large for the streamed kernel's LDS budget");
   }
@@ -1135,7 +1211,7 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
   if ((rc = upload(P->inst, &P->d_inst)) || (rc = upload(P->items, &P->d_items)) ||
       (rc = upload(rowptr, &P->d_rowptr)) || (rc = upload(ent, &P->d_ent)) || (rc = upload(coef, &P->d_coef)) ||
       (rc = upload(mode, &P->d_mode)) || (rc = upload(mat, &P->d_mat)) || (rc = upload(P->resident, &P->d_res)) ||
-      (rc = upload(prowptr, &P->d_prowptr)) || (rc = upload(pent, &P->d_pent)) || (rc = upload(pgs, &P->d_pgs)) ||
+      (rc = upload(sites, &P->d_site)) ||
       (rc = upload(coefp, &P->d_coefp))) {
     destroy_plan(P);
     return rc;
@@ -1257,9 +1333,9 @@ StageParams base_params(const pyqed_heom_plan* P) {
   p.mode = P->d_mode;
   p.mat = P->d_mat;
   p.resident = P->d_res;
-  p.p_rowptr = P->d_prowptr;
-  p.p_ent = P->d_pent;
-  p.p_gs = P->d_pgs;
+  p.site = P->d_site;
+  p.rowbuf = P->rowbuf;
+  p.colbuf = P->colbuf;
   p.coefp = P->d_coefp;
   p.traj_stride = P->traj_stride;
   p.d2max = P->d2max;
@@ -1299,15 +1375,22 @@ int launch_stage(const pyqed_heom_plan* P, StageParams p, hipStream_t st) {
     std::memcpy(p.xr, P->xr[c], sizeof(p.xr));
     const int grid = stream_grid(P, n);
     int rc;
-    if (c == 0) {
+    // the projector kernel keeps neighbour rows (and columns) in LDS; when they do not fit, the
+    // generic kernel (which any instance can use) takes the class
+    if (c == 0 || proj_lds_bytes(P, sizeof(T)) > (size_t)P->max_lds) {
       rc = launch(heom_stage_kernel<T, TI, T, S>, grid, stream_lds_bytes(P, sizeof(T)), st, p);
-    } else if (c == 2) {
-      if constexpr (std::is_same<T, float>::value)
-        rc = launch(heom_proj_kernel<T, TI, T, S, true>, grid, proj_lds_bytes(P, sizeof(T)), st, p);
+    } else if (P->hermitian) {
+      if (c == 2 && std::is_same<T, float>::value)
+        rc = launch(heom_proj_kernel<T, TI, T, S, std::is_same<T, float>::value, true>, grid, proj_lds_bytes(P, sizeof(T)),
+                    st, p);
       else
-        rc = launch(heom_proj_kernel<T, TI, T, S, false>, grid, proj_lds_bytes(P, sizeof(T)), st, p);
+        rc = launch(heom_proj_kernel<T, TI, T, S, false, true>, grid, proj_lds_bytes(P, sizeof(T)), st, p);
     } else {
-      rc = launch(heom_proj_kernel<T, TI, T, S, false>, grid, proj_lds_bytes(P, sizeof(T)), st, p);
+      if (c == 2 && std::is_same<T, float>::value)
+        rc = launch(heom_proj_kernel<T, TI, T, S, std::is_same<T, float>::value, false>, grid,
+                    proj_lds_bytes(P, sizeof(T)), st, p);
+      else
+        rc = launch(heom_proj_kernel<T, TI, T, S, false, false>, grid, proj_lds_bytes(P, sizeof(T)), st, p);
     }
     if (rc) return rc;
   }

@@ -76,10 +76,23 @@ def prepare_instance(inst):
     return out
 
 
+def _herm(m, tol=1e-14):
+    return np.abs(m - m.conj().T).max(initial=0.0) <= tol * (1.0 + np.abs(m).max(initial=0.0))
+
+
+def _hermitian_dynamics(p):
+    """True when every ADO stays Hermitian: Hermitian H, dipole and rho0, real diagonal couplings,
+    no coupling dipole, real exponents and etar = conj(etal) for every exponential."""
+    etal, etar, _, expn = p["bath"]
+    return (_herm(p["H"]) and (p["sdip"] is None or _herm(p["sdip"])) and _herm(p["rho0"]) and p["Cmat"] is None
+            and np.all(np.imag(p["qdiag"]) == 0) and np.all(np.imag(expn) == 0)
+            and np.all(np.abs(etar - np.conj(etal)) <= 1e-14 * np.abs(etal)))
+
+
 class HeomEngine:
     """A plan over one or more HEOM instances plus its device state."""
 
-    def __init__(self, instances, *, resident=True, generic=False):
+    def __init__(self, instances, *, resident=True, generic=False, hermitian=None):
         self.torch = torch = _lib.require_gpu()
         self.L = _lib.lib()
         self.prep = [prepare_instance(i) for i in instances]
@@ -105,8 +118,11 @@ class HeomEngine:
             s.d, s.K, s.L, s.ncoup = d, len(p["mode"]), p["L"], p["qdiag"].shape[0]
             for name, b in bufs.items():
                 setattr(s, name, None if b is None else b.ctypes.data)
+        if hermitian is None:
+            hermitian = not self.dense and all(_hermitian_dynamics(p) for p in self.prep)
+        self.hermitian = bool(hermitian)
         flags = ((0 if resident else _lib.PLAN_NO_RESIDENT) | (_lib.PLAN_DENSE_Q if self.dense else 0) |
-                 (_lib.PLAN_GENERIC if generic else 0))
+                 (_lib.PLAN_GENERIC if generic else 0) | (_lib.PLAN_HERMITIAN if self.hermitian else 0))
         plan = C.c_void_p()
         _lib.check(self.L.pyqed_heom_plan_create(arr, self.n, flags, C.byref(plan)))
         self.plan = plan
