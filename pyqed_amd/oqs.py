@@ -1,0 +1,145 @@
+"""Drop-in Lindblad and Redfield solvers (pyqed/oqs.py) on the MI355X Liouville-space kernels.
+
+LindbladSolver(H, c_ops, e_ops).run(rho0, dt, Nt, t0=0., e_ops=None, return_result=True) and
+RedfieldSolver(H, c_ops, spectra, e_ops).evolve(rho0, dt, Nt, ...) return the reference's Result
+(observables, rholist, times).  The RK4 is that of pyqed/phys.py:1051-1063 and runs on the GPU.
+
+Documented differences from the reference:
+  * the state is dense: the reference's _lindblad converts rho to csr and, with numpy 2, the
+    coherent term -i[H, rho] then evaluates to an array of empty matrices (lost); here it is kept;
+  * a driven Hamiltonian H = [H0, [H1, f1], ...] means H0 - sum_i f_i(t) H_i at t = t0 + (k+1) dt
+    for step k (the reference's evaluation point, oqs.py:1792-1796) without the reference's in-place
+    accumulation of every past drive into H0 (oqs.py:1725-1732); run(..., legacy_accumulate=True)
+    reproduces that accumulation;
+  * rholist holds dense numpy arrays.
+"""
+import numpy as np
+
+from .liouville import dag, lindblad_generator, op2sop, propagate
+from .mol import Result
+
+
+class LindbladSolver:
+    def __init__(self, H=None, c_ops=None, e_ops=None):
+        self.c_ops = c_ops
+        self.e_ops = e_ops
+        self.H = H
+
+    def set_c_ops(self, c_ops):
+        self.c_ops = c_ops
+
+    def set_e_ops(self, e_ops):
+        self.e_ops = e_ops
+
+    def setH(self, H):
+        self.H = H
+
+    def configure(self, c_ops, e_ops):
+        self.c_ops = c_ops
+        self.e_ops = e_ops
+
+    def liouvillian(self):
+        """Liouvillian superoperator (superoperator.liouvillian), row-major vec."""
+        return lindblad_generator(self.H, self.c_ops or [])
+
+    def run(self, rho0, dt, Nt, t0=0.0, e_ops=None, return_result=True, *, legacy_accumulate=False):
+        e_ops = [] if e_ops is None else list(e_ops)
+        c_ops = self.c_ops or []
+        rho0 = np.asarray(rho0, np.complex128)
+        result = Result(dt=dt, Nt=Nt, rho0=rho0)
+        if isinstance(self.H, list):
+            H0 = np.asarray(self.H[0], np.complex128)
+            terms = self.H[1:]
+            gens = [lindblad_generator(H0, c_ops)] + [1j * op2sop(np.asarray(Hi, np.complex128)) for Hi, _ in terms]
+            field = np.zeros((Nt, len(terms)))
+            for k in range(Nt):
+                t = t0 + (k + 1) * dt
+                for i, (_, fi) in enumerate(terms):
+                    v = complex(fi(t))
+                    if v.imag != 0.0:
+                        raise NotImplementedError("complex-valued drive amplitudes are not supported")
+                    field[k, i] = v.real
+            if legacy_accumulate:
+                field = np.cumsum(field, axis=0)
+            obs, states, _ = propagate(np.array(gens), rho0[None], dt, Nt, field[None], e_ops)
+            result.observables = obs[0, 1:]
+        else:
+            gens = lindblad_generator(np.asarray(self.H, np.complex128), c_ops)[None]
+            obs, states, _ = propagate(gens, rho0[None], dt, Nt, None, e_ops)
+            result.observables = obs[0]
+        result.rholist = [states[0, k] for k in range(states.shape[1])]
+        return result
+
+    def run_batch(self, rho0s, dt, Nt, fields=None):
+        """Propagate many initial states (and, for a driven H, per-instance fields [n, Nt, n_terms])
+        in one launch; returns the final states [n, d, d]."""
+        c_ops = self.c_ops or []
+        rho0s = np.asarray(rho0s, np.complex128)
+        if isinstance(self.H, list):
+            H0 = np.asarray(self.H[0], np.complex128)
+            gens = [lindblad_generator(H0, c_ops)] + [1j * op2sop(np.asarray(Hi, np.complex128)) for Hi, _ in self.H[1:]]
+        else:
+            gens = [lindblad_generator(np.asarray(self.H, np.complex128), c_ops)]
+        _, _, final = propagate(np.array(gens), rho0s, dt, Nt, fields, (), keep_states=False)
+        return final
+
+
+def redfield_tensor(H, a_ops, spectra, secular=False):
+    """Bloch-Redfield generator in the eigenbasis of H (oqs.py:519-570); returns (R, evecs) with R a
+    dense N x N matrix such that d vec(rho)/dt = R vec(rho).  `secular` is accepted and ignored, as
+    in the reference."""
+    for a in a_ops:
+        a = a.toarray() if hasattr(a, "toarray") else np.asarray(a)
+        if not np.allclose(a, dag(a)):
+            raise TypeError("Operators in a_ops must be Hermitian.")
+    H = H.toarray() if hasattr(H, "toarray") else np.asarray(H)
+    evals, evecs = np.linalg.eigh(H)
+    W = np.real(evals[:, None] - evals[None, :])
+    N = len(evals)
+    R = 0
+    for a, s in zip(a_ops, spectra):
+        a = a.toarray() if hasattr(a, "toarray") else np.asarray(a, np.complex128)
+        C = np.array([[s(-W[n, m]) for m in range(N)] for n in range(N)], dtype=float)
+        A = dag(evecs) @ a @ evecs
+        Lk = C * A
+        R = R + op2sop(A) @ (op2sop(Lk, "left") - op2sop(dag(Lk), "right"))
+    return -1j * op2sop(np.diag(evals)) - R, evecs
+
+
+class RedfieldSolver:
+    def __init__(self, H, c_ops=None, spectra=None, e_ops=None):
+        self.H = H
+        self.c_ops = c_ops
+        self.R = None
+        self.spectra = spectra
+        self.evecs = None
+        self.dim = H.shape[0]
+        self.e_ops = e_ops
+
+    def configure(self, H, c_ops, e_ops):
+        self.c_ops = c_ops
+        self.e_ops = e_ops
+        self.H = H
+
+    def redfield_tensor(self, secular=False):
+        if self.spectra is None:
+            raise TypeError("Specify the bath spectral function.")
+        R, evecs = redfield_tensor(self.H, self.c_ops, self.spectra, secular)
+        self.R = R
+        self.evecs = evecs
+        return R, evecs
+
+    def evolve(self, rho0, dt, Nt, evecs=None, e_ops=[], store_states=False, t0=0, nout=1):
+        """RK4 on vec(rho) in the eigenbasis (oqs.py:57-81, 364-459): observables [Nt, n_e] in the
+        eigenbasis (no t0 row), rholist back-transformed to the original basis."""
+        if self.R is None:
+            self.redfield_tensor()
+        ev = self.evecs
+        rho0 = np.asarray(rho0, np.complex128)
+        e_eig = [dag(ev) @ np.asarray(e, np.complex128) @ ev for e in (e_ops or [])]
+        obs, states, _ = propagate(np.asarray(self.R, np.complex128)[None], (dag(ev) @ rho0 @ ev)[None], dt, Nt,
+                                   None, e_eig)
+        result = Result(dt=dt, Nt=Nt, rho0=rho0)
+        result.observables = obs[0, 1:]
+        result.rholist = [ev @ states[0, k] @ dag(ev) for k in range(states.shape[1])]
+        return result
