@@ -26,6 +26,8 @@ class LindbladAdjoint:
         self.B = np.asarray(field_gens, np.complex128).reshape(-1, *self.base.shape)
         self.N = self.base.shape[0]
         self.d = int(round(np.sqrt(self.N)))
+        if self.d * self.d != self.N:
+            raise ValueError("generator dimension must be a perfect square")
         self.dt = float(dt)
         self.sps = int(steps_per_seg)
         self.nseg = int(nseg)
@@ -92,6 +94,37 @@ def lambda_system(eps, mu, c_ops, dt, steps_per_seg, nseg, ckpt_every=50):
         G.append(-1j * op2sop(P))
     B = [1j * op2sop(np.asarray(mu, np.complex128))]
     return LindbladAdjoint(base, G, B, dt, steps_per_seg, nseg, ckpt_every)
+
+
+def heom_problem(H, Q, bath, mode, L, dt, steps_per_seg, nseg, param_ops=(), field_ops=(), ckpt_every=50):
+    """HEOM gradients for small hierarchies (nmax*d^2 <= 512): the dense HEOM generator (sparse
+    assembly from the library's tables) propagated by the Liouville-space kernels.
+    H(t) = H + sum_j theta_j P_j + sum_f E_f(t) M_f (the HEOM sign, deom.py:676-687); theta
+    multiplies param_ops P_j, the field amplitudes multiply field_ops M_f.  The state is the whole
+    hierarchy; the reduced density matrix is its first d*d block."""
+    from .heom.generator import heom_generator
+    H = np.asarray(H, np.complex128)
+    d = H.shape[0]
+    base = heom_generator(H, Q, bath, mode, L, sparse=False)
+    nmax = base.shape[0] // (d * d)
+    if base.shape[0] > 512:
+        raise ValueError("heom_problem handles nmax*d^2 <= 512 (the Liouville adjoint kernel's LDS limit)")
+    eye = np.eye(nmax)
+    G = [np.kron(eye, -1j * op2sop(np.asarray(P, np.complex128))) for P in param_ops]
+    B = [np.kron(eye, -1j * op2sop(np.asarray(M, np.complex128))) for M in field_ops]
+    prob = LindbladAdjoint(base, np.array(G) if G else np.zeros((0,) + base.shape),
+                           np.array(B) if B else np.zeros((0,) + base.shape), dt, steps_per_seg, nseg, ckpt_every)
+    prob.hier_d = d
+    return prob
+
+
+def embed_rho0(prob, rho0):
+    """Hierarchy initial state with rho0 in ADO 0 (for heom_problem)."""
+    v = np.zeros(prob.N, np.complex128)
+    d = prob.hier_d
+    v[:d * d] = np.asarray(rho0, np.complex128).reshape(-1)
+    n = int(round(np.sqrt(prob.N)))
+    return v.reshape(n, n)
 
 
 def _autograd_function():

@@ -150,6 +150,60 @@ class DEOMSolver:
         P1 = np.asarray(p1, np.complex128)
         return t_save, np.einsum("ij,tji->t", P1, traj)
 
+    # ---------------------------------------------------------------- generator and spectra
+    def gen_generate_propgator(self):
+        """Dense generator self.propgator, vec(RHS(rho)) = propgator vec(rho), row-major ADO blocks
+        (deom.py:1116-1125, 769-884), assembled from the hierarchy tables."""
+        from .generator import heom_generator
+        self.check_()
+        self.init_()
+        b = self.bath
+        Q = [np.asarray(q, np.complex128) for q in self.coupling]
+        self.propgator = heom_generator(np.asarray(self.system, np.complex128), Q,
+                                        (b.etal, b.etar, b.etaa, b.expn), b.mode, self.lmax, sparse=False)
+        return self.propgator
+
+    def correlation_4op_3t(self, operator_a, operator_b, operator_c, operator_d, rho0, T, w_x, w_y, if_full=True,
+                           cut_off_min=0.5, cut_off_max=1.1, if_load=False, if_save=False, lcr="llll"):
+        """Two-frequency four-point correlation (deom.py:1127-1210):
+        C(wx, wy) = Tr_0[ d G(wx) c e^{L T} b G(wy) a rho0 ],  G(w) = (-L - i w)^-1,
+        with operator x acting as lcr[.] ('l' x., 'r' .x, 'c' x. + .x; a uses lcr[0], d lcr[3]),
+        evaluated in the eigenbasis of the dense generator L; with if_full=False only eigenvalues with
+        cut_off_min*min(Re) < Re < cut_off_max*max(Re) are kept."""
+        import os
+        import scipy.linalg as la
+        from .generator import system_action
+        if getattr(self, "propgator", None) is None:
+            self.gen_generate_propgator()
+        if if_load and os.path.exists("correlation_4op_3t.npz"):
+            data = np.load("correlation_4op_3t.npz")
+            self._delta, self._V, self._Vinv = data["Δ"], data["V"], data["V_inv"]
+        if getattr(self, "_delta", None) is None:
+            self._delta, self._V = la.eig(self.propgator)
+            self._Vinv = la.pinv(self._V)
+        if if_save:
+            np.savez("correlation_4op_3t.npz", Δ=self._delta, V=self._V, V_inv=self._Vinv)
+        n = self.nsys
+        act = {name: system_action(op, self.nmax, c).toarray()
+               for name, op, c in zip("abcd", (operator_a, operator_b, operator_c, operator_d), lcr)}
+        rho = np.zeros(self.nmax * n * n, complex)
+        rho[:n * n] = np.asarray(rho0, complex).reshape(-1)
+        lam, V, Vi = self._delta, self._V, self._Vinv
+        if not if_full:
+            re = np.real(lam)
+            keep = (re > np.min(re) * cut_off_min) & (re < np.max(re) * cut_off_max)
+            lam, V, Vi = lam[keep], V[:, keep], Vi[keep, :]
+        left = (act["d"] @ V)[:n * n]                                              # d V (ADO 0 rows)
+        mid = ((Vi @ act["c"] @ V) * np.exp(lam * T)[None, :]) @ (Vi @ act["b"] @ V)  # c e^{LT} b
+        right = Vi @ (act["a"] @ rho)                                              # a rho
+        out = np.zeros((len(w_x), len(w_y)), complex)
+        for i, wx in enumerate(w_x):
+            A = left * (1.0 / (-lam - 1j * wx))[None, :]
+            for j, wy in enumerate(w_y):
+                v = A @ (mid @ ((1.0 / (-lam - 1j * wy)) * right))
+                out[i, j] = np.trace(v.reshape(n, n))
+        return out
+
     @property
     def ddos(self):
         """Final hierarchy state [nmax, d, d] of the last run (the reference's self.ddos)."""
