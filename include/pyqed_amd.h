@@ -142,6 +142,12 @@ int pyqed_heom_rhs(pyqed_heom_plan* plan, const void* x, void* out, const double
  * streamed work items, resident instances, streamed elements, resident
  * elements. */
 int pyqed_heom_plan_stats(const pyqed_heom_plan* plan, int64_t* stats4);
+/* Work-queue tiling: target complex elements per work item (0 = automatic)
+ * and persistent-grid blocks per CU of the streamed kernels.  Used by the
+ * autotuner (pyqed_amd.heom.autotune); no counterpart in the reference.
+ * set_tiling synchronises the device before replacing the queue. */
+int pyqed_heom_plan_set_tiling(pyqed_heom_plan* plan, int64_t target_elems, int32_t blocks_per_cu);
+int pyqed_heom_plan_get_tiling(const pyqed_heom_plan* plan, int64_t* target_elems, int32_t* blocks_per_cu);
 /* Streamed work per kernel class (0 generic, 1 site-projector with VALU
  * commutator, 2 site-projector with MFMA commutator): items and elements
  * (host arrays [3], either may be NULL). */

@@ -8,7 +8,8 @@ import torch
 
 
 def final_state(theta, amps, base, G, B, rho0, dt, steps_per_seg):
-    """rho(T) [d, d] as a torch complex128 tensor; theta [P] and amps [F, nseg] are float64 tensors."""
+    """rho(T) as a torch complex128 tensor ([d, d] when the generator is d^2 x d^2, else the state
+    vector); theta [P] and amps [F, nseg] are float64 tensors."""
     base = torch.as_tensor(np.asarray(base, np.complex128))
     G = torch.as_tensor(np.asarray(G, np.complex128).reshape(-1, *base.shape))
     B = torch.as_tensor(np.asarray(B, np.complex128).reshape(-1, *base.shape))
@@ -24,4 +25,4 @@ def final_state(theta, amps, base, G, B, rho0, dt, steps_per_seg):
             k4 = A @ (v + k3 * dt)
             v = v + (k1 + 2 * k2 + 2 * k3 + k4) / 6.0 * dt
     d = int(round(np.sqrt(v.shape[0])))
-    return v.reshape(d, d)
+    return v.reshape(d, d) if d * d == v.shape[0] else v

@@ -26,8 +26,8 @@ class LindbladAdjoint:
         self.B = np.asarray(field_gens, np.complex128).reshape(-1, *self.base.shape)
         self.N = self.base.shape[0]
         self.d = int(round(np.sqrt(self.N)))
-        if self.d * self.d != self.N:
-            raise ValueError("generator dimension must be a perfect square")
+        # a d x d density matrix when N = d^2, otherwise a plain state vector (e.g. a whole hierarchy)
+        self.shape = (self.d, self.d) if self.d * self.d == self.N else (self.N,)
         self.dt = float(dt)
         self.sps = int(steps_per_seg)
         self.nseg = int(nseg)
@@ -53,10 +53,10 @@ class LindbladAdjoint:
         _lib.check(L.pyqed_liouville_run(self.N, 1 + len(self.B), A.data_ptr(), 1, rho.data_ptr(), _lib.ptr(f),
                                          self.nsteps, self.dt, 0, None, None, None, 1, _lib.stream_ptr(torch)))
         torch.cuda.synchronize()
-        return rho.cpu().numpy().reshape(self.d, self.d)
+        return rho.cpu().numpy().reshape(self.shape)
 
     def gradient(self, theta, amps, rho0, rhoT_bar):
-        """(dJ/dtheta [P], dJ/damps [F, nseg], dJ/drho0 [d, d]) for rhoT_bar = dJ/dRe rho(T) + i dJ/dIm rho(T)."""
+        """(dJ/dtheta [P], dJ/damps [F, nseg], dJ/drho0) for rhoT_bar = dJ/dRe rho(T) + i dJ/dIm rho(T)."""
         torch = _lib.require_gpu()
         L = _lib.lib()
         dev = torch.device("cuda", torch.cuda.current_device())
@@ -80,7 +80,7 @@ class LindbladAdjoint:
         g_theta = np.array([np.real(np.sum(np.conj(gA0) * Gj)) for Gj in self.G])
         g_steps = gf[0].cpu().numpy()[:, :nterm - 1]
         g_amps = g_steps.reshape(self.nseg, self.sps, -1).sum(axis=1).T
-        return g_theta, g_amps, r0b[0].cpu().numpy().reshape(self.d, self.d)
+        return g_theta, g_amps, r0b[0].cpu().numpy().reshape(self.shape)
 
 
 def lambda_system(eps, mu, c_ops, dt, steps_per_seg, nseg, ckpt_every=50):
@@ -119,12 +119,11 @@ def heom_problem(H, Q, bath, mode, L, dt, steps_per_seg, nseg, param_ops=(), fie
 
 
 def embed_rho0(prob, rho0):
-    """Hierarchy initial state with rho0 in ADO 0 (for heom_problem)."""
+    """Hierarchy initial state (vector [nmax*d*d]) with rho0 in ADO 0 (for heom_problem)."""
     v = np.zeros(prob.N, np.complex128)
     d = prob.hier_d
     v[:d * d] = np.asarray(rho0, np.complex128).reshape(-1)
-    n = int(round(np.sqrt(prob.N)))
-    return v.reshape(n, n)
+    return v.reshape(prob.shape)
 
 
 def _autograd_function():

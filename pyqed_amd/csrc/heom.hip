@@ -902,6 +902,9 @@ struct pyqed_heom_plan {
   int32_t* d_site = nullptr;
   double2* d_coefp = nullptr;
   int n_proj = 0;
+  std::vector<char> is_res;
+  int64_t tile_target = 0;
+  int blocks_per_cu = 8;
   int rowbuf = 1, colbuf = 0;  // projector path: max entries*d over instances (columns: non-Hermitian)
   bool hermitian = false;
   hipStream_t side = nullptr;
@@ -962,6 +965,52 @@ void destroy_plan(pyqed_heom_plan* P) {
 }
 
 std::complex<double> cz(const double* p, int i) { return std::complex<double>(p[2 * i], p[2 * i + 1]); }
+
+// Cut every instance into work items of about `target` complex elements (0: automatic), streamed
+// classes first (generic, projector/VALU, projector/MFMA), resident instances last, and split each
+// class into eight contiguous ranges of equal element count (one per XCD).  `blocks_per_cu` sets the
+// persistent grid of the streamed kernels.
+void retile(pyqed_heom_plan* P, int64_t target, int blocks_per_cu) {
+  if (target <= 0)
+    target = std::min<int64_t>(8192, std::max<int64_t>(256, P->stream_elems / std::max<int64_t>(1, (int64_t)P->num_cu * 8)));
+  P->tile_target = target;
+  P->blocks_per_cu = std::max(1, std::min(blocks_per_cu, 32));
+  P->items.clear();
+  auto tile = [&](int i) {
+    const InstDev& I = P->inst[i];
+    const int nslot = (I.path == 0 || P->is_res[i]) ? kThreads / I.grp : kThreads / 64;
+    int64_t per = (target + I.d2 - 1) / I.d2;
+    per = ((per + nslot - 1) / nslot) * nslot;
+    per = std::max<int64_t>(per, nslot);
+    for (int a = 0; a < I.nmax; a += (int)per) {
+      WorkItem w;
+      w.inst = i;
+      w.ado0 = a;
+      w.nado = (int)std::min<int64_t>(per, I.nmax - a);
+      w.pad = 0;
+      P->items.push_back(w);
+    }
+  };
+  for (int c = 0; c < 3; ++c) {
+    P->cls_begin[c] = (int)P->items.size();
+    for (int i = 0; i < P->n_inst; ++i)
+      if (!P->is_res[i] && P->inst[i].path == c) tile(i);
+  }
+  P->cls_begin[3] = P->n_items_stream = (int)P->items.size();
+  for (int i : P->resident) tile(i);
+  for (int c = 0; c < 3; ++c) {
+    const int b0 = P->cls_begin[c], n = P->cls_begin[c + 1] - b0;
+    std::vector<int64_t> pre(n + 1, 0);
+    for (int k = 0; k < n; ++k) pre[k + 1] = pre[k] + (int64_t)P->items[b0 + k].nado * P->inst[P->items[b0 + k].inst].d2;
+    P->xr[c][0] = 0;
+    for (int x = 1; x < 8; ++x) {
+      const int64_t goal = pre.back() * x / 8;
+      P->xr[c][x] = (int)(std::lower_bound(pre.begin(), pre.end(), goal) - pre.begin());
+      P->xr[c][x] = std::max(P->xr[c][x], P->xr[c][x - 1]);
+    }
+    P->xr[c][8] = n;
+  }
+}
 
 }  // namespace
 
@@ -1161,47 +1210,10 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
   }
   // ---- work items: tiles of ADOs, streamed instances first
   for (int i = 0; i < n_inst; ++i) (is_res[i] ? P->res_elems : P->stream_elems) += (int64_t)P->inst[i].nmax * P->inst[i].d2;
-  const int64_t target =
-      std::min<int64_t>(8192, std::max<int64_t>(256, P->stream_elems / std::max<int64_t>(1, (int64_t)P->num_cu * 8)));
-  auto tile = [&](int i) {
-    const InstDev& I = P->inst[i];
-    const int nslot = (I.path == 0 || is_res[i]) ? kThreads / I.grp : kThreads / 64;
-    int64_t per = (target + I.d2 - 1) / I.d2;
-    per = ((per + nslot - 1) / nslot) * nslot;
-    per = std::max<int64_t>(per, nslot);
-    for (int a = 0; a < I.nmax; a += (int)per) {
-      WorkItem w;
-      w.inst = i;
-      w.ado0 = a;
-      w.nado = (int)std::min<int64_t>(per, I.nmax - a);
-      w.pad = 0;
-      P->items.push_back(w);
-    }
-  };
-  for (int c = 0; c < 3; ++c) {
-    P->cls_begin[c] = (int)P->items.size();
-    for (int i = 0; i < n_inst; ++i)
-      if (!is_res[i] && P->inst[i].path == c) tile(i);
-  }
-  P->cls_begin[3] = P->n_items_stream = (int)P->items.size();
+  P->is_res = is_res;
   for (int i = 0; i < n_inst; ++i)
-    if (is_res[i]) {
-      tile(i);
-      P->resident.push_back(i);
-    }
-  // XCD ranges of each class, balanced by element count
-  for (int c = 0; c < 3; ++c) {
-    const int b0 = P->cls_begin[c], n = P->cls_begin[c + 1] - b0;
-    std::vector<int64_t> pre(n + 1, 0);
-    for (int k = 0; k < n; ++k) pre[k + 1] = pre[k] + (int64_t)P->items[b0 + k].nado * P->inst[P->items[b0 + k].inst].d2;
-    P->xr[c][0] = 0;
-    for (int x = 1; x < 8; ++x) {
-      const int64_t goal = pre.back() * x / 8;
-      P->xr[c][x] = (int)(std::lower_bound(pre.begin(), pre.end(), goal) - pre.begin());
-      P->xr[c][x] = std::max(P->xr[c][x], P->xr[c][x - 1]);
-    }
-    P->xr[c][8] = n;
-  }
+    if (is_res[i]) P->resident.push_back(i);
+  retile(P, 0, 8);
   if (stream_lds_bytes(P, sizeof(double)) > (size_t)P->max_lds) {
     destroy_plan(P);
     return fail(PYQED_EUNSUPPORTED, "instance too large for the streamed kernel's LDS budget");
@@ -1257,6 +1269,22 @@ extern "C" int pyqed_heom_plan_stats(const pyqed_heom_plan* P, int64_t* s) {
   s[1] = (int64_t)P->resident.size();
   s[2] = P->stream_elems;
   s[3] = P->res_elems;
+  return PYQED_OK;
+}
+
+extern "C" int pyqed_heom_plan_set_tiling(pyqed_heom_plan* P, int64_t target_elems, int32_t blocks_per_cu) {
+  if (!P || blocks_per_cu < 1 || target_elems < 0) return fail(PYQED_EINVAL, "plan_set_tiling: bad argument");
+  PQA_HIP(hipDeviceSynchronize());  // the old queue may still be in use by queued launches
+  retile(P, target_elems, blocks_per_cu);
+  (void)hipFree(P->d_items);
+  P->d_items = nullptr;
+  return upload(P->items, &P->d_items);
+}
+
+extern "C" int pyqed_heom_plan_get_tiling(const pyqed_heom_plan* P, int64_t* target_elems, int32_t* blocks_per_cu) {
+  if (!P) return fail(PYQED_EINVAL, "plan_get_tiling: null plan");
+  if (target_elems) *target_elems = P->tile_target;
+  if (blocks_per_cu) *blocks_per_cu = P->blocks_per_cu;
   return PYQED_OK;
 }
 
@@ -1356,7 +1384,7 @@ int launch(Kern k, int grid, size_t lds, hipStream_t s, const StageParams& p) {
 }
 
 int stream_grid(const pyqed_heom_plan* P, int n_items) {
-  int g = std::min(n_items, P->num_cu * 8);
+  int g = std::min(n_items, P->num_cu * P->blocks_per_cu);
   g = ((g + 7) / 8) * 8;
   return std::max(g, 8);
 }

@@ -1,0 +1,95 @@
+"""Autotuner for the HEOM work queue: chooses the work-item size (complex elements per item) and
+the persistent grid (blocks per CU) of the streamed stage kernels by timing RK4 steps.
+
+Results are cached per workload signature (instances' (d, K, L), Hermitian mode, precision, device
+name) in pyqed_amd/tuning.json; HeomEngine applies a cached entry before its first step.
+
+    python -m pyqed_amd.heom.autotune --workload c3 [--out tuning.json]
+"""
+import argparse
+import hashlib
+import json
+import os
+import time
+
+import numpy as np
+
+from .. import _lib
+
+TUNING_FILE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tuning.json")
+TARGETS = (1024, 2048, 4096, 8192, 16384)
+BLOCKS_PER_CU = (2, 4, 8, 16)
+
+
+def signature(engine, precision):
+    torch = engine.torch
+    key = [torch.cuda.get_device_name(engine.device), precision, engine.hermitian, engine.dense,
+           [(p["d"], len(p["mode"]), p["L"]) for p in engine.prep]]
+    return hashlib.sha1(json.dumps(key).encode()).hexdigest()[:20]
+
+
+def load_cache(path=TUNING_FILE):
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def lookup(engine, precision, path=TUNING_FILE):
+    e = load_cache(path).get(signature(engine, precision))
+    return None if e is None else (int(e["target_elems"]), int(e["blocks_per_cu"]))
+
+
+def set_tiling(engine, target, bpc):
+    _lib.check(engine.L.pyqed_heom_plan_set_tiling(engine.plan, int(target), int(bpc)))
+
+
+def tune(engine, dt, precision="mixed", targets=TARGETS, blocks_per_cu=BLOCKS_PER_CU, steps=2, warmup=1):
+    """Time every candidate on `engine` (its state advances); leaves the best tiling applied and
+    returns {"best": (target, bpc), "timings": {...}} with milliseconds per step."""
+    torch = engine.torch
+    timings = {}
+    for t in targets:
+        for b in blocks_per_cu:
+            set_tiling(engine, t, b)
+            engine.advance(warmup, dt, precision)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            engine.advance(steps, dt, precision)
+            torch.cuda.synchronize()
+            timings[f"{t}x{b}"] = (time.perf_counter() - t0) / steps * 1e3
+    best = min(timings, key=timings.get)
+    bt, bb = (int(x) for x in best.split("x"))
+    set_tiling(engine, bt, bb)
+    return {"best": (bt, bb), "timings": timings}
+
+
+def record(engine, precision, result, path=TUNING_FILE):
+    cache = load_cache(path)
+    cache[signature(engine, precision)] = {"target_elems": result["best"][0], "blocks_per_cu": result["best"][1],
+                                           "ms_per_step": result["timings"][f"{result['best'][0]}x{result['best'][1]}"],
+                                           "instances": engine.n, "ados": int(engine.n_ados)}
+    with open(path, "w") as f:
+        json.dump(cache, f, indent=1, sort_keys=True)
+
+
+def main():
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))))
+    import bench
+    from .engine import HeomEngine
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--n-inst", type=int, default=4096)
+    ap.add_argument("--precision", default="mixed")
+    ap.add_argument("--out", default=TUNING_FILE)
+    a = ap.parse_args()
+    insts, dt, _ = bench.instances(a.workload, a.n_inst)
+    eng = HeomEngine(insts)
+    res = tune(eng, dt, a.precision)
+    record(eng, a.precision, res, a.out)
+    print(json.dumps({"workload": a.workload, "precision": a.precision, **res}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

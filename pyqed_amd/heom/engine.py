@@ -140,6 +140,8 @@ class HeomEngine:
         self.y = torch.zeros(self.n_elems, dtype=torch.complex128, device=self.device)
         self._ws = None
         self._ws_prec = None
+        self._tuned = set()
+        self.autotune_cache = True
         self.step = 0
         self.reset()
 
@@ -191,6 +193,12 @@ class HeomEngine:
         [>= step+nsteps, 6]; `traj` a device complex128 tensor [nsteps//save_every, traj_stride]."""
         torch = self.torch
         prec = PRECISIONS[precision]
+        if self.autotune_cache and precision not in self._tuned:
+            from . import autotune
+            self._tuned.add(precision)
+            hit = autotune.lookup(self, precision)
+            if hit is not None:
+                autotune.set_tiling(self, *hit)
         ws = self.workspace(prec)
         _lib.check(self.L.pyqed_heom_run(
             self.plan, prec, self.y.data_ptr(), ws.data_ptr(), ws.numel(),
