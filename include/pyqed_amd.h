@@ -148,6 +148,10 @@ int pyqed_heom_plan_stats(const pyqed_heom_plan* plan, int64_t* stats4);
  * set_tiling synchronises the device before replacing the queue. */
 int pyqed_heom_plan_set_tiling(pyqed_heom_plan* plan, int64_t target_elems, int32_t blocks_per_cu);
 int pyqed_heom_plan_get_tiling(const pyqed_heom_plan* plan, int64_t* target_elems, int32_t* blocks_per_cu);
+/* Kernel variant of the MFMA class for Hermitian FP32 runs: 0 = one ADO per
+ * wavefront at a time, 1 = two-deep software pipeline.  Chosen by the
+ * autotuner; results are identical. */
+int pyqed_heom_plan_set_variant(pyqed_heom_plan* plan, int32_t variant);
 /* Streamed work per kernel class (0 generic, 1 site-projector with VALU
  * commutator, 2 site-projector with MFMA commutator): items and elements
  * (host arrays [3], either may be NULL). */

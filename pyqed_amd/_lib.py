@@ -23,7 +23,7 @@ SYMBOLS = [
     "pyqed_last_error", "pyqed_version", "pyqed_device_count",
     "pyqed_heom_plan_create", "pyqed_heom_plan_destroy", "pyqed_heom_plan_query", "pyqed_heom_plan_layout",
     "pyqed_heom_tables_export", "pyqed_heom_tables_size", "pyqed_heom_tables_build", "pyqed_heom_run", "pyqed_heom_rhs", "pyqed_heom_plan_stats",
-    "pyqed_heom_plan_classes", "pyqed_heom_plan_set_tiling", "pyqed_heom_plan_get_tiling", "pyqed_liouville_run", "pyqed_liouville_adjoint",
+    "pyqed_heom_plan_classes", "pyqed_heom_plan_set_tiling", "pyqed_heom_plan_get_tiling", "pyqed_heom_plan_set_variant", "pyqed_liouville_run", "pyqed_liouville_adjoint",
 ]
 
 
@@ -73,6 +73,7 @@ def lib():
     L.pyqed_heom_plan_classes.argtypes = [vp, vp, vp]
     L.pyqed_heom_plan_set_tiling.argtypes = [vp, i64, i32]
     L.pyqed_heom_plan_get_tiling.argtypes = [vp, C.POINTER(i64), C.POINTER(i32)]
+    L.pyqed_heom_plan_set_variant.argtypes = [vp, i32]
     L.pyqed_liouville_run.argtypes = [i32, i32, vp, i32, vp, vp, i64, dbl, i32, vp, vp, vp, i64, vp]
     L.pyqed_liouville_adjoint.argtypes = [i32, i32, vp, i32, vp, vp, i64, dbl, i32, vp, vp, vp, vp, vp, i64, vp]
     for name in SYMBOLS:

@@ -679,6 +679,260 @@ __global__ __launch_bounds__(kThreads, PQA_PROJ_WAVES) void heom_proj_kernel(Sta
   }
 }
 
+// ------------------------------------------------------------------ pipelined site-projector kernel
+// The dominant class (12 <= d <= 16, Hermitian hierarchy, FP32 right-hand side) in a two-deep
+// software pipeline: while ADO a is gathered and written, the neighbour rows of ADO a+1 are in
+// flight and the neighbour list and matrix of ADO a+2 are being fetched.  Per-wave LDS buffers
+// (own matrix, neighbour list, row buffer) are doubled.  Loads are issued in the order
+//   rows(a+1) ... y/ynew(a) ... rows(a+2) ... list/matrix(a+3)
+// relative to where they are waited for, so the in-order vector-memory counter never makes a wait
+// for one ADO's data wait for a later ADO's.
+#ifndef PQA_PIPE_WAVES
+#define PQA_PIPE_WAVES 2
+#endif
+
+__host__ __device__ inline size_t pipe_wave_bytes(int d2max, int rowbuf) {
+  size_t b = 2 * (8 * ((size_t)d2max + rowbuf) + 2 * 8 * kMaxEnt + 2 * 4 * kMaxEnt + 4 * 36);
+  return (b + 15) & ~size_t(15);
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(kThreads, PQA_PIPE_WAVES) void heom_pipe_kernel(StageParams p) {
+  using T = float;
+  using TI = typename std::conditional<STAGE == 0, double, float>::type;
+  extern __shared__ __align__(16) char smem[];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  cpx<T>* sH = (cpx<T>*)smem;
+  cpx<T>* sC = sH + p.d2max;
+  T* sS = (T*)(sC + 5 * p.Kmax);
+  int32_t* sSite = (int32_t*)(sS + p.Lmax + 2);
+  char* wb = smem + proj_block_bytes(p.d2max, p.Kmax, p.Lmax, sizeof(T)) + wv * pipe_wave_bytes(p.d2max, p.rowbuf);
+  // two buffers of everything per wave
+  cpx<T>* Xb[2];
+  cpx<T>* Rb[2];
+  cpx<T>* clb[2];
+  cpx<T>* crb[2];
+  int32_t* nbb[2];
+  int32_t* stb[2];
+  int32_t* gsb[2];
+  {
+    char* q = wb;
+    for (int u = 0; u < 2; ++u) {
+      Xb[u] = (cpx<T>*)q;
+      q += 8 * (size_t)p.d2max;
+      Rb[u] = (cpx<T>*)q;
+      q += 8 * (size_t)p.rowbuf;
+      clb[u] = (cpx<T>*)q;
+      q += 8 * kMaxEnt;
+      crb[u] = (cpx<T>*)q;
+      q += 8 * kMaxEnt;
+      nbb[u] = (int32_t*)q;
+      q += 4 * kMaxEnt;
+      stb[u] = (int32_t*)q;
+      q += 4 * kMaxEnt;
+      gsb[u] = (int32_t*)q;
+      q += 4 * 36;
+    }
+  }
+  const cpx_store<TI>* xin = (const cpx_store<TI>*)p.xin;
+  cpx_store<T>* xout = (cpx_store<T>*)p.xout;
+  const int col = p.col_override >= 0 ? p.col_override : stage_col(STAGE & 3);
+  int it0, itstep, itend;
+  item_range(p, it0, itend, itstep);
+  for (int it = it0; it < itend; it += itstep) {
+    const WorkItem wi = p.items[it];
+    const InstDev I = p.insts[wi.inst];
+    const int d = I.d, d2 = I.d2, rs = d + 1;
+    const uint32_t dmag = ((1u << 20) + d - 1) / d;
+    __syncthreads();
+    {
+      double fs = 0.0;
+      if (p.field) fs = p.field[6 * p.step + col];
+      const double2* H = p.mat + I.h_off;
+      const bool sd = (I.flags & F_SDIP) != 0;
+      for (int e = tid; e < d2; e += kThreads) {
+        double2 h = H[e];
+        if (sd) {
+          const double2 s2 = H[d2 + e];
+          h.x += s2.x * fs;
+          h.y += s2.y * fs;
+        }
+        sH[e] = mk<T>((T)h.x, (T)h.y);
+      }
+      for (int e = tid; e < 5 * I.K; e += kThreads) {
+        const double2 v = p.coefp[I.coefp_off + e];
+        sC[e] = mk<T>((T)v.x, (T)v.y);
+      }
+      for (int e = tid; e < I.K; e += kThreads) sSite[e] = p.site[I.site_off + e];
+      for (int e = tid; e < I.L + 2; e += kThreads) sS[e] = (T)sqrt((double)e);
+    }
+    __syncthreads();
+    const int gi = lane & 15, gk = lane >> 4;
+    const cpx_store<TI>* xi = xin + I.off;
+    const int aend = wi.ado0 + wi.nado, astep = kThreads / 64;
+    const int afirst = wi.ado0 + wv;
+    if (afirst >= aend) continue;
+    // ---- pipeline state in registers
+    int nent_n = 0;
+    int2 en_n = make_int2(0, 0);
+    cpx_store<TI> xt_n[4];
+    cpx_store<TI> v[kChunk];
+    int nval_n = 0;  // number of row values in flight for the next ADO
+    auto fetch_list = [&](int an) {
+      const int r0 = p.rowptr[I.rowptr_off + an];
+      nent_n = p.rowptr[I.rowptr_off + an + 1] - r0;
+      en_n = make_int2(0, 0);
+      if (lane < nent_n) en_n = p.ent[I.ent_off + r0 + lane];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = lane + 64 * q;
+        if (e < d2) xt_n[q] = xi[(int64_t)an * d2 + e];
+      }
+    };
+    // sort the fetched list into buffer u, copy the fetched matrix into X[u]; returns damping
+    auto prep = [&](int u) -> cpx<T> {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = lane + 64 * q;
+        if (e < d2) Xb[u][e] = cvt<T>(mk(xt_n[q].x, xt_n[q].y));
+      }
+      const int nent = nent_n;
+      const int k = en_n.y & 0xffff, nf = (en_n.y >> 16) & 0x7f;
+      const bool plus = en_n.y < 0;
+      const int my_site = lane < nent ? sSite[k] : 0x7fffffff;
+      int pos = 0, gsv = 0, run = 0;
+      const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+      for (int s2 = 0; s2 < d; ++s2) {
+        const uint64_t m = __ballot(my_site == s2);
+        if (lane == s2) gsv = run;
+        if (my_site == s2) pos = run + __popcll(m & below);
+        run += __popcll(m);
+      }
+      if (lane == d) gsv = run;
+      if (lane <= d) gsb[u][lane] = gsv;
+      cpx<T> dp = mk<T>(0, 0);
+      if (lane < nent) {
+        const cpx<T>* C = sC + 5 * k;
+        const T sq = sS[nf];
+        nbb[u][pos] = en_n.x;
+        stb[u][pos] = my_site;
+        clb[u][pos] = (plus ? C[3] : C[1]) * sq;
+        crb[u][pos] = (plus ? C[4] : C[2]) * sq;
+        if (!plus) dp = C[0] * (T)nf;
+      }
+      nval_n = nent * d;
+      return wave_sum(dp);
+    };
+    auto issue_rows = [&](int u) {
+#pragma unroll
+      for (int q = 0; q < kChunk; ++q) {
+        const int idx = lane + 64 * q;
+        if (idx < nval_n) {
+          const int t = (int)(((uint32_t)idx * dmag) >> 20), jj = idx - t * d;
+          v[q] = xi[(int64_t)nbb[u][t] * d2 + stb[u][t] * d + jj];
+        }
+      }
+    };
+    auto stash_rows = [&](int u, int nval) {
+#pragma unroll
+      for (int q = 0; q < kChunk; ++q) {
+        const int idx = lane + 64 * q;
+        if (idx < nval) {
+          const int t = (int)(((uint32_t)idx * dmag) >> 20);
+          Rb[u][idx + t] = cvt<T>(mk(v[q].x, v[q].y));
+        }
+      }
+      // values beyond the first chunk (more than 64*kChunk) are fetched and stored directly
+      for (int q0 = 64 * kChunk; q0 < nval; q0 += 64) {
+        const int idx = q0 + lane;
+        if (idx < nval) {
+          const int t = (int)(((uint32_t)idx * dmag) >> 20), jj = idx - t * d;
+          const cpx_store<TI> w2 = xi[(int64_t)nbb[u][t] * d2 + stb[u][t] * d + jj];
+          Rb[u][idx + t] = cvt<T>(mk(w2.x, w2.y));
+        }
+      }
+    };
+    // ---- prologue: ADO afirst
+    fetch_list(afirst);
+    cpx<T> damp = prep(0);
+    int nval = nval_n;
+    wave_sync();
+    issue_rows(0);
+    if (afirst + astep < aend) fetch_list(afirst + astep);
+    int u = 0;
+    for (int a = afirst; a < aend; a += astep, u ^= 1) {
+      const int64_t base = (int64_t)a * d2;
+      // commutator of ADO a (its matrix is in X[u])
+      floatx4 aR = {0.f, 0.f, 0.f, 0.f}, aI = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kc = 0; kc < 4; ++kc) {
+        const int kk = 4 * kc + gk;
+        const bool ok = gi < d && kk < d;
+        const cpx<T> xb = ok ? Xb[u][kk * d + gi] : mk<T>(0, 0);
+        const cpx<T> xa = ok ? Xb[u][gi * d + kk] : mk<T>(0, 0);
+        const cpx<T> ha = ok ? sH[gi * d + kk] : mk<T>(0, 0);
+        const cpx<T> hb = ok ? sH[kk * d + gi] : mk<T>(0, 0);
+        aR = __builtin_amdgcn_mfma_f32_16x16x4f32(ha.x, xb.x, aR, 0, 0, 0);
+        aR = __builtin_amdgcn_mfma_f32_16x16x4f32(-ha.y, xb.y, aR, 0, 0, 0);
+        aI = __builtin_amdgcn_mfma_f32_16x16x4f32(ha.x, xb.y, aI, 0, 0, 0);
+        aI = __builtin_amdgcn_mfma_f32_16x16x4f32(ha.y, xb.x, aI, 0, 0, 0);
+        aR = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.x, -hb.x, aR, 0, 0, 0);
+        aR = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.y, hb.y, aR, 0, 0, 0);
+        aI = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.x, -hb.y, aI, 0, 0, 0);
+        aI = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.y, -hb.x, aI, 0, 0, 0);
+      }
+      cpx<T> acc[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 4 * gk + r;
+        acc[r] = mul_mi(mk<T>(aR[r], aI[r]));
+        if (i < d && gi < d) cmsub(acc[r], damp, Xb[u][i * d + gi]);
+      }
+      // rows of ADO a arrive; then its RK4 operands are requested
+      stash_rows(u, nval);
+      Pre pre[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (4 * gk + r < d && gi < d) pre[r] = rk4_prefetch<STAGE>(p, I.off + base + (4 * gk + r) * d + gi);
+      wave_sync();
+      // next ADO: sort its list, request its rows, fetch the list after it
+      cpx<T> damp_n = mk<T>(0, 0);
+      const bool has_next = a + astep < aend;
+      if (has_next) {
+        damp_n = prep(u ^ 1);
+        wave_sync();
+        issue_rows(u ^ 1);
+        if (a + 2 * astep < aend) fetch_list(a + 2 * astep);
+      }
+      // gather ADO a's neighbour terms from its row buffer and write its update
+      const bool save_ado = STAGE == 3 && a == 0 && p.traj && p.save_slot >= 0;
+      const int64_t tbase = p.save_slot * p.traj_stride + I.traj_off;
+      if (gi < d) {
+        for (int t = gsb[u][gi]; t < gsb[u][gi + 1]; ++t) {
+          const cpx<T> c = crb[u][t];
+          const cpx<T>* row = Rb[u] + t * rs + 4 * gk;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (4 * gk + r < d) cmac(acc[r], c, conjg(row[r]));
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 4 * gk + r;
+          if (i < d) {
+            for (int t = gsb[u][i]; t < gsb[u][i + 1]; ++t) cmac(acc[r], clb[u][t], Rb[u][t * rs + gi]);
+            const int e = i * d + gi;
+            rk4_store<T, STAGE>(p, I.off + base + e, (double)acc[r].x, (double)acc[r].y, pre[r], xout, save_ado,
+                                tbase + e);
+          }
+        }
+      }
+      damp = damp_n;
+      nval = nval_n;
+      wave_sync();
+    }
+  }
+}
+
 // ------------------------------------------------------------------ LDS-resident kernel
 // One workgroup owns one small instance for the whole run: y, ynew and both
 // stage buffers live in LDS, so HBM is touched only to load the initial state,
@@ -905,6 +1159,7 @@ struct pyqed_heom_plan {
   std::vector<char> is_res;
   int64_t tile_target = 0;
   int blocks_per_cu = 8;
+  int variant = 0;  // 0: site-projector kernel; 1: pipelined kernel for the MFMA class (Hermitian, FP32)
   int rowbuf = 1, colbuf = 0;  // projector path: max entries*d over instances (columns: non-Hermitian)
   bool hermitian = false;
   hipStream_t side = nullptr;
@@ -933,6 +1188,10 @@ size_t stream_lds_bytes(const pyqed_heom_plan* P, size_t tsize) {
 size_t proj_lds_bytes(const pyqed_heom_plan* P, size_t tsize) {
   return proj_block_bytes(P->d2max, P->Kmax, P->Lmax, tsize) +
          (kThreads / 64) * proj_wave_bytes(P->d2max, P->rowbuf, P->colbuf, tsize);
+}
+
+size_t pipe_lds_bytes(const pyqed_heom_plan* P) {
+  return proj_block_bytes(P->d2max, P->Kmax, P->Lmax, sizeof(float)) + (kThreads / 64) * pipe_wave_bytes(P->d2max, P->rowbuf);
 }
 
 size_t resident_lds_bytes(const pyqed_heom_plan* P, int64_t n_el, size_t tsize) {
@@ -1281,6 +1540,12 @@ extern "C" int pyqed_heom_plan_set_tiling(pyqed_heom_plan* P, int64_t target_ele
   return upload(P->items, &P->d_items);
 }
 
+extern "C" int pyqed_heom_plan_set_variant(pyqed_heom_plan* P, int32_t variant) {
+  if (!P || variant < 0 || variant > 1) return fail(PYQED_EINVAL, "plan_set_variant: variant must be 0 or 1");
+  P->variant = variant;
+  return PYQED_OK;
+}
+
 extern "C" int pyqed_heom_plan_get_tiling(const pyqed_heom_plan* P, int64_t* target_elems, int32_t* blocks_per_cu) {
   if (!P) return fail(PYQED_EINVAL, "plan_get_tiling: null plan");
   if (target_elems) *target_elems = P->tile_target;
@@ -1377,7 +1642,18 @@ StageParams base_params(const pyqed_heom_plan* P) {
 
 template <typename Kern>
 int launch(Kern k, int grid, size_t lds, hipStream_t s, const StageParams& p) {
-  if (lds > 65536) PQA_HIP(hipFuncSetAttribute(kernel_ptr(k), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  // raise the kernel's dynamic-LDS limit only when a larger size is first needed, so launches that
+  // are being captured into a graph make no attribute calls
+  static std::mutex mu;
+  static std::map<void*, size_t> limit;
+  if (lds > 65536) {
+    std::lock_guard<std::mutex> g(mu);
+    size_t& cur = limit[kernel_ptr(k)];
+    if (lds > cur) {
+      PQA_HIP(hipFuncSetAttribute(kernel_ptr(k), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      cur = lds;
+    }
+  }
   hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), lds, s, p);
   PQA_HIP(hipGetLastError());
   return PYQED_OK;
@@ -1407,6 +1683,9 @@ int launch_stage(const pyqed_heom_plan* P, StageParams p, hipStream_t st) {
     // generic kernel (which any instance can use) takes the class
     if (c == 0 || proj_lds_bytes(P, sizeof(T)) > (size_t)P->max_lds) {
       rc = launch(heom_stage_kernel<T, TI, T, S>, grid, stream_lds_bytes(P, sizeof(T)), st, p);
+    } else if (P->hermitian && c == 2 && std::is_same<T, float>::value && P->variant == 1 &&
+               pipe_lds_bytes(P) <= (size_t)P->max_lds) {
+      rc = launch(heom_pipe_kernel<S>, grid, pipe_lds_bytes(P), st, p);
     } else if (P->hermitian) {
       if (c == 2 && std::is_same<T, float>::value)
         rc = launch(heom_proj_kernel<T, TI, T, S, std::is_same<T, float>::value, true>, grid, proj_lds_bytes(P, sizeof(T)),

@@ -17,8 +17,9 @@ import numpy as np
 from .. import _lib
 
 TUNING_FILE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tuning.json")
-TARGETS = (1024, 2048, 4096, 8192, 16384)
-BLOCKS_PER_CU = (2, 4, 8, 16)
+TARGETS = (2048, 4096, 8192, 16384)
+BLOCKS_PER_CU = (4, 8, 16)
+VARIANTS = (0, 1)
 
 
 def signature(engine, precision):
@@ -37,37 +38,42 @@ def load_cache(path=TUNING_FILE):
 
 def lookup(engine, precision, path=TUNING_FILE):
     e = load_cache(path).get(signature(engine, precision))
-    return None if e is None else (int(e["target_elems"]), int(e["blocks_per_cu"]))
+    return None if e is None else (int(e["target_elems"]), int(e["blocks_per_cu"]), int(e.get("variant", 0)))
 
 
-def set_tiling(engine, target, bpc):
+def set_tiling(engine, target, bpc, variant=0):
     _lib.check(engine.L.pyqed_heom_plan_set_tiling(engine.plan, int(target), int(bpc)))
+    _lib.check(engine.L.pyqed_heom_plan_set_variant(engine.plan, int(variant)))
 
 
-def tune(engine, dt, precision="mixed", targets=TARGETS, blocks_per_cu=BLOCKS_PER_CU, steps=2, warmup=1):
-    """Time every candidate on `engine` (its state advances); leaves the best tiling applied and
-    returns {"best": (target, bpc), "timings": {...}} with milliseconds per step."""
+def tune(engine, dt, precision="mixed", targets=TARGETS, blocks_per_cu=BLOCKS_PER_CU, variants=VARIANTS, steps=2,
+         warmup=1):
+    """Time every candidate (work-item size x blocks per CU x kernel variant) on `engine` (its state
+    advances); leaves the best applied and returns {"best": (target, bpc, variant), "timings": {...}}
+    with milliseconds per step."""
     torch = engine.torch
     timings = {}
-    for t in targets:
-        for b in blocks_per_cu:
-            set_tiling(engine, t, b)
-            engine.advance(warmup, dt, precision)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            engine.advance(steps, dt, precision)
-            torch.cuda.synchronize()
-            timings[f"{t}x{b}"] = (time.perf_counter() - t0) / steps * 1e3
+    for v in variants:
+        for t in targets:
+            for b in blocks_per_cu:
+                set_tiling(engine, t, b, v)
+                engine.advance(warmup, dt, precision)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                engine.advance(steps, dt, precision)
+                torch.cuda.synchronize()
+                timings[f"{t}x{b}x{v}"] = (time.perf_counter() - t0) / steps * 1e3
     best = min(timings, key=timings.get)
-    bt, bb = (int(x) for x in best.split("x"))
-    set_tiling(engine, bt, bb)
-    return {"best": (bt, bb), "timings": timings}
+    bt, bb, bv = (int(x) for x in best.split("x"))
+    set_tiling(engine, bt, bb, bv)
+    return {"best": (bt, bb, bv), "timings": timings}
 
 
 def record(engine, precision, result, path=TUNING_FILE):
     cache = load_cache(path)
-    cache[signature(engine, precision)] = {"target_elems": result["best"][0], "blocks_per_cu": result["best"][1],
-                                           "ms_per_step": result["timings"][f"{result['best'][0]}x{result['best'][1]}"],
+    bt, bb, bv = result["best"]
+    cache[signature(engine, precision)] = {"target_elems": bt, "blocks_per_cu": bb, "variant": bv,
+                                           "ms_per_step": result["timings"][f"{bt}x{bb}x{bv}"],
                                            "instances": engine.n, "ados": int(engine.n_ados)}
     with open(path, "w") as f:
         json.dump(cache, f, indent=1, sort_keys=True)

@@ -206,6 +206,44 @@ class HeomEngine:
             None if traj is None else traj.data_ptr(), save_every, _lib.stream_ptr(torch, stream)))
         self.step += nsteps
 
+    def advance_graph(self, nsteps, dt, precision="fp64", chunk=25):
+        """Advance nsteps steps by replaying a captured HIP graph of `chunk` steps (the stage
+        launches of every kernel class, and the LDS-resident launch on its side stream).  Without a
+        time-dependent field no kernel reads the step index, so one captured chunk serves every
+        replay.  The remainder runs eagerly.  Returns the number of graph replays."""
+        torch = self.torch
+        key = (precision, chunk, float(dt))
+        if getattr(self, "_graphs", None) is None:
+            self._graphs = {}
+        g = self._graphs.get(key)
+        if g is None:
+            step0 = self.step
+            self.workspace(PRECISIONS[precision])
+            if self.autotune_cache and precision not in self._tuned:
+                self.advance(0, dt, precision)  # apply cached tiling before capture
+            y0 = self.y.clone()
+            s = torch.cuda.Stream()
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                self.advance(chunk, dt, precision, stream=s)  # warm up on the capture stream
+            torch.cuda.current_stream().wait_stream(s)
+            self.y.copy_(y0)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self.advance(chunk, dt, precision, stream=torch.cuda.current_stream())
+            self.y.copy_(y0)  # capture does not execute; restore anyway in case a backend did
+            del y0
+            self.step = step0
+            self._graphs[key] = g
+        nrep = nsteps // chunk
+        for _ in range(nrep):
+            g.replay()
+        rest = nsteps - nrep * chunk
+        if rest:
+            self.advance(rest, dt, precision)
+        self.step += nrep * chunk
+        return nrep
+
     def run(self, dt, nt, precision="fp64", field=None, save_every=1):
         """Propagate nt steps from the current state; returns host trajectories of rho per instance,
         each [nt//save_every + 1, d, d] (row 0 = the state before the call), in the caller's basis."""

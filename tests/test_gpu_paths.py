@@ -125,3 +125,24 @@ def test_mixed_batch_of_all_paths():
         err = np.abs(out[k] - ref).max()
         print(f"instance d={inst['H'].shape[0]}: mixed batched vs fp64 generic {err:.2e}")
         assert err <= 1e-6
+
+
+@pytest.mark.parametrize("stage_prec", ["mixed"])
+def test_pipelined_variant_bitwise_equal(stage_prec):
+    """The two-deep pipelined kernel computes the same arithmetic in the same order as the base
+    projector kernel: identical states after several steps, across many work items."""
+    import torch
+    from pyqed_amd import _lib
+    from pyqed_amd.heom.engine import HeomEngine
+    insts = [_instance(d, L, seed=200 + d) for d in (12, 14, 16) for L in (2, 3)]
+    outs = []
+    for variant in (0, 1):
+        eng = HeomEngine(insts, resident=False)
+        eng.autotune_cache = False
+        _lib.check(eng.L.pyqed_heom_plan_set_tiling(eng.plan, 1024, 4))
+        _lib.check(eng.L.pyqed_heom_plan_set_variant(eng.plan, variant))
+        assert eng.hermitian and eng.classes()["items"][2] > 0
+        eng.advance(3, 0.02, stage_prec)
+        torch.cuda.synchronize()
+        outs.append(eng.y.clone())
+    assert torch.equal(outs[0], outs[1])
