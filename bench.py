@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-fp64-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", type=int, default=0, metavar="CHUNK",
+                    help="time graph replays of CHUNK captured steps instead of eager launches")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (scale reference for HBM byte counters)")
     args = ap.parse_args()
@@ -161,16 +163,28 @@ def main():
     eng.advance(args.warmup, dt, args.precision)
     torch.cuda.synchronize()
     print("[bench] warmup done", file=sys.stderr, flush=True)
+    if args.graph:
+        eng.advance_graph(args.graph, dt, args.precision, chunk=args.graph)  # capture (+ one replay) untimed
+        torch.cuda.synchronize()
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     evs[0].record()
-    for s in range(args.steps):
-        eng.advance(1, dt, args.precision)
-        evs[s + 1].record()
+    if args.graph:
+        for s in range(0, args.steps, args.graph):
+            eng.advance_graph(min(args.graph, args.steps - s), dt, args.precision, chunk=args.graph)
+        evs[-1].record()
+    else:
+        for s in range(args.steps):
+            eng.advance(1, dt, args.precision)
+            evs[s + 1].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    step_ms = [evs[s].elapsed_time(evs[s + 1]) for s in range(args.steps)]
+    if args.graph:
+        step_ms = [evs[0].elapsed_time(evs[-1]) / args.steps]
+    else:
+        step_ms = [evs[s].elapsed_time(evs[s + 1]) for s in range(args.steps)]
+    tiling = eng.tiling()
     value = eng.n_ados * args.steps / wall
     print(f"[bench] timed {args.steps} steps in {wall:.3f}s", file=sys.stderr, flush=True)
     bpe = BYTES_MIXED if args.precision == "mixed" else BYTES_FP64
@@ -192,7 +206,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "heom_stage_kernel (4 RK4-stage launches per step, streamed instances)",
                      "bytes_per_step": bpe * st["stream_elems"], "step_ms_events": float(np.mean(step_ms))},
-        "plan_build_s": t_plan, "plan": st,
+        "plan_build_s": t_plan, "plan": st, "tiling": tiling, "graph_chunk": args.graph or None,
     }
     from math import comb
     out["config"]["ados"] = int(sum(comb(len(i["bath"][4]) + i["L"], i["L"]) for i in insts))

@@ -642,7 +642,9 @@ __global__ __launch_bounds__(kThreads, PQA_PROJ_WAVES) void heom_proj_kernel(Sta
             const cpx<T>* row = (HERM ? w.R : w.Cc) + t * rs + 4 * gk;
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-              if (4 * gk + r < d) cmac(acc[r], c, HERM ? conjg(row[r]) : row[r]);
+              // the diagonal element keeps X'(s, s) itself, so that (cL + cR) X'(s, s) is formed exactly
+              // as without the Hermitian shortcut (cL = -cR for the n + e_k terms: no trace drift)
+              if (4 * gk + r < d) cmac(acc[r], c, (HERM && 4 * gk + r != gi) ? conjg(row[r]) : row[r]);
           }
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
@@ -670,7 +672,7 @@ __global__ __launch_bounds__(kThreads, PQA_PROJ_WAVES) void heom_proj_kernel(Sta
           cmsub(o, damp, w.X[e]);
           for (int t = w.gs[i]; t < w.gs[i + 1]; ++t) cmac(o, w.cl[t], w.R[t * rs + j]);
           for (int t = w.gs[j]; t < w.gs[j + 1]; ++t)
-            cmac(o, w.cr[t], HERM ? conjg(w.R[t * rs + i]) : w.Cc[t * rs + i]);
+            cmac(o, w.cr[t], HERM ? (i != j ? conjg(w.R[t * rs + i]) : w.R[t * rs + i]) : w.Cc[t * rs + i]);
           rk4_store<TO, STAGE>(p, I.off + base + e, (double)o.x, (double)o.y, pr, xout, save_ado, tbase + e);
         }
       }
@@ -913,7 +915,7 @@ __global__ __launch_bounds__(kThreads, PQA_PIPE_WAVES) void heom_pipe_kernel(Sta
           const cpx<T>* row = Rb[u] + t * rs + 4 * gk;
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            if (4 * gk + r < d) cmac(acc[r], c, conjg(row[r]));
+            if (4 * gk + r < d) cmac(acc[r], c, 4 * gk + r != gi ? conjg(row[r]) : row[r]);
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {

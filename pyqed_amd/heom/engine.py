@@ -141,6 +141,7 @@ class HeomEngine:
         self._ws = None
         self._ws_prec = None
         self._tuned = set()
+        self.variant = 0
         self.autotune_cache = True
         self.step = 0
         self.reset()
@@ -175,6 +176,11 @@ class HeomEngine:
             o = int(self.state_offset[k])
             self.y[o:o + a.size] = torch.as_tensor(a.reshape(-1), device=self.device)
 
+    def tiling(self):
+        t, b = C.c_int64(), C.c_int32()
+        _lib.check(self.L.pyqed_heom_plan_get_tiling(self.plan, C.byref(t), C.byref(b)))
+        return {"target_elems": t.value, "blocks_per_cu": b.value, "variant": self.variant}
+
     def classes(self):
         it = np.zeros(3, np.int64)
         el = np.zeros(3, np.int64)
@@ -199,6 +205,7 @@ class HeomEngine:
             hit = autotune.lookup(self, precision)
             if hit is not None:
                 autotune.set_tiling(self, *hit)
+                self.variant = hit[2]
         ws = self.workspace(prec)
         _lib.check(self.L.pyqed_heom_run(
             self.plan, prec, self.y.data_ptr(), ws.data_ptr(), ws.numel(),

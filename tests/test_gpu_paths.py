@@ -146,3 +146,25 @@ def test_pipelined_variant_bitwise_equal(stage_prec):
         torch.cuda.synchronize()
         outs.append(eng.y.clone())
     assert torch.equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("d", [8, 16])
+def test_hermitian_mode_conserves_trace_like_exact_path(d):
+    """Regression: in Hermitian mode the diagonal element must use X'(s, s) itself; conj(X'(s, s))
+    turns the rounding-level imaginary part of a neighbour's diagonal into a real contribution to
+    Tr rho_0 (c_L + c_R = 0 for the n + e_k terms only with the unconjugated value)."""
+    from pyqed_amd.heom.engine import HeomEngine
+    inst = _instance(d, 2, seed=d + 40, weight=1.0)
+    rng = np.random.default_rng(d + 3)
+    probe = HeomEngine([inst], resident=False)
+    nmax = int(probe.nmax[0])
+    A = (rng.normal(size=(nmax, d, d)) + 1j * rng.normal(size=(nmax, d, d))) * 0.1
+    state = A + np.conj(np.transpose(A, (0, 2, 1)))
+    state[:, np.arange(d), np.arange(d)] += 1e-3j  # a non-Hermitian residue on every diagonal
+    traces = {}
+    for herm in (True, False):
+        eng = HeomEngine([inst], resident=False, hermitian=herm)
+        eng.set_state([state])
+        eng.advance(1, 0.05, "fp64")
+        traces[herm] = np.trace(eng.hierarchy(0)[0])
+    assert abs(traces[True] - traces[False]) <= 1e-13, traces
