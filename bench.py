@@ -119,6 +119,40 @@ def cpu_baseline(insts, dt, budget_s=15.0):
                        f"{ados} ADO-updates in {t_used:.1f} s")
 
 
+def stream_probe(torch):
+    """Achievable HBM bandwidth on this device (GB/s): torch device copies of 1, 8 and 32 GiB, and a
+    stream with the byte mix of a middle RK4 stage (read complex128 ynew, y and complex64 X; write
+    complex128 ynew and complex64 X'), as two fused torch additions over 2^30 elements."""
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+    out = {}
+    for gib in (1, 8, 32):
+        n = gib << 30
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        out[f"copy_{gib}GiB_GBps"] = 2 * n / timed(lambda: b.copy_(a)) / 1e9
+        del a, b
+    n = 1 << 30
+    y = torch.zeros(n, dtype=torch.complex128, device="cuda")
+    yn = torch.zeros_like(y)
+    x = torch.zeros(n, dtype=torch.complex64, device="cuda")
+    x2 = torch.zeros_like(x)
+
+    def stage():
+        torch.add(yn, x, alpha=0.5, out=yn)   # read ynew, X; write ynew
+        torch.add(y, yn, alpha=0.25, out=x2)  # read y, ynew; write X'
+    out["stage_mix_GBps"] = n * (16 + 8 + 16 + 16 + 16 + 8) / timed(stage) / 1e9
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +167,8 @@ def main():
                     help="time graph replays of CHUNK captured steps instead of eager launches")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (scale reference for HBM byte counters)")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="only measure achievable HBM bandwidth: device copies and a 3-read/2-write stream")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the propagator has no multi-GPU path "
@@ -149,6 +185,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n}), flush=True)
+        return
+    if args.stream_probe:
+        print(json.dumps(stream_probe(torch)), flush=True)
         return
 
     insts, dt, desc = instances(args.workload, args.n_inst)

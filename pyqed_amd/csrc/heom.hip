@@ -67,7 +67,25 @@ struct WorkItem {
   int32_t inst, ado0, nado, pad;
 };
 
-enum { F_SDIP = 1, F_CDIP = 2, F_DENSE = 4, F_HERM = 8 };
+enum { F_SDIP = 1, F_CDIP = 2, F_DENSE = 4, F_HERM = 8, F_SITESORT = 16 };  // F_SITESORT: site(k) nondecreasing
+
+// Plan tables (instances, work items, row pointers) are never written while a kernel runs. Read
+// through the constant address space at a wave-uniform index they become scalar loads, which
+// are counted apart from vector loads: waiting for one does not drain the vector loads in flight.
+#define PQA_CONST __attribute__((address_space(4)))
+
+template <typename V>
+__device__ __forceinline__ V ro_load(const V* q) {
+  static_assert(sizeof(V) % 4 == 0, "word-sized records only");
+  const PQA_CONST uint32_t* s = (const PQA_CONST uint32_t*)q;
+  V v;
+  uint32_t* w = (uint32_t*)&v;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(V) / 4); ++i) w[i] = s[i];
+  return v;
+}
+
+__device__ __forceinline__ int32_t ro_int(const int32_t* q, int64_t i) { return ((const PQA_CONST int32_t*)q)[i]; }
 
 struct StageParams {
   const InstDev* insts;
@@ -98,6 +116,7 @@ struct StageParams {
   int32_t col_override;  // >= 0: field column used instead of the stage's own (RHS-only evaluation)
   int32_t rowbuf, colbuf;  // projector path: per-wave neighbour row / column buffer (complex elements)
   int32_t d2max, qmax, Kmax, Lmax;
+  int32_t d2pad;  // projector path: max d*(d+1), matrices in LDS with row stride d+1
 };
 
 // LDS carve-up shared by both kernels.
@@ -406,7 +425,19 @@ constexpr int kMaxEnt = 64;  // neighbour entries per ADO handled by the project
 #ifndef PQA_PROJ_WAVES
 #define PQA_PROJ_WAVES 3  // minimum waves per SIMD the projector kernel is compiled for
 #endif
-constexpr int kChunk = 8;    // neighbour loads per lane issued together
+#ifndef PQA_VALU_WAVES
+#define PQA_VALU_WAVES 4  // the same for its FP32 VALU-commutator form (small d, fewer registers)
+#endif
+#ifndef PQA_CHUNK
+#define PQA_CHUNK 2
+#endif
+constexpr int kChunk = PQA_CHUNK;  // neighbour loads per lane issued together
+#ifndef PQA_RK4_HOIST
+// MFMA path, RK4 operands of a lane's four elements: 0 one request per element at its update,
+// 1 requested together after the neighbour rows have arrived, 2 requested together right behind
+// the neighbour-row requests (both round trips overlap the commutator)
+#define PQA_RK4_HOIST 1
+#endif
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -424,6 +455,32 @@ __device__ __forceinline__ cpx<T> wave_sum(cpx<T> v) {
   return v;
 }
 
+// Site order of a wave's neighbour list (one entry per lane, my_site = INT_MAX past the end):
+// returns the entry's position in site order and, in lanes 0..d, the start of each site's group.
+// The tables list entries by exponential k (minus, then plus), so when site(k) is nondecreasing
+// (F_SITESORT) the list is already in site order and only the group starts are counted; otherwise
+// a stable counting sort over ballots orders it.  Both keep the table order within a site.
+__device__ __forceinline__ int site_order(int my_site, int d, int lane, bool presorted, int& gsv) {
+  gsv = 0;
+  if (presorted) {
+    for (int s2 = 0; s2 <= d; ++s2) {
+      const int c = __popcll(__ballot(my_site < s2));
+      if (lane == s2) gsv = c;
+    }
+    return lane;
+  }
+  int pos = 0, run = 0;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int s2 = 0; s2 < d; ++s2) {
+    const uint64_t m = __ballot(my_site == s2);
+    if (lane == s2) gsv = run;
+    if (my_site == s2) pos = run + __popcll(m & below);
+    run += __popcll(m);
+  }
+  if (lane == d) gsv = run;
+  return pos;
+}
+
 template <typename T>
 struct WaveLds {
   cpx<T>* X;    // own ADO, d*d
@@ -436,29 +493,61 @@ struct WaveLds {
   int32_t* gs;  // site-group starts, d+1
 };
 
-__host__ __device__ inline size_t proj_wave_bytes(int d2max, int rowbuf, int colbuf, size_t ts) {
-  size_t b = 2 * ts * ((size_t)d2max + rowbuf + colbuf) + 2 * 2 * ts * kMaxEnt + 2 * 4 * kMaxEnt + 4 * 36;
+__host__ __device__ inline size_t proj_wave_bytes(int xsize, int rowbuf, int colbuf, size_t ts) {
+  size_t b = 2 * ts * ((size_t)xsize + rowbuf + colbuf) + 2 * 2 * ts * kMaxEnt + 2 * 4 * kMaxEnt + 4 * 36;
   return (b + 15) & ~size_t(15);
 }
-__host__ __device__ inline size_t proj_block_bytes(int d2max, int Kmax, int Lmax, size_t ts) {
-  size_t b = 2 * ts * (size_t)d2max + 2 * ts * 5 * (size_t)Kmax + ts * (size_t)(Lmax + 2) + 4 * (size_t)Kmax;
+__host__ __device__ inline size_t proj_block_bytes(int xsize, int Kmax, int Lmax, size_t ts) {
+  size_t b = 2 * ts * (size_t)xsize + 2 * ts * 5 * (size_t)Kmax + ts * (size_t)(Lmax + 2) + 4 * (size_t)Kmax;
   return (b + 15) & ~size_t(15);
 }
 
+// Per-item tables of the projector kernels in LDS: H (+ field x system dipole) with row stride d+1,
+// the projector-scaled coefficients, the site of every exponential and sqrt(0..L+1).  One pass
+// issues all of its loads before the first LDS store waits for them.
+template <typename T>
+__device__ __forceinline__ void proj_header(const StageParams& p, const InstDev& I, int col, uint32_t dmag,
+                                            cpx<T>* sH, cpx<T>* sC, int32_t* sSite, T* sS) {
+  const double fs = p.field ? ro_load(p.field + 6 * p.step + col) : 0.0;
+  const double2* H = p.mat + I.h_off;
+  const bool sd = (I.flags & F_SDIP) != 0;
+  const int d2 = I.d2, nc = 5 * I.K, n = max(max(d2, nc), I.L + 2);
+  for (int e = threadIdx.x; e < n; e += kThreads) {
+    double2 h = make_double2(0, 0), s2 = make_double2(0, 0), c = make_double2(0, 0);
+    int32_t st = 0;
+    if (e < d2) h = H[e];
+    if (sd && e < d2) s2 = H[d2 + e];
+    if (e < nc) c = p.coefp[I.coefp_off + e];
+    if (e < I.K) st = p.site[I.site_off + e];
+    if (e < d2) {
+      if (sd) {
+        h.x += s2.x * fs;
+        h.y += s2.y * fs;
+      }
+      sH[e + (int)(((uint32_t)e * dmag) >> 20)] = mk<T>((T)h.x, (T)h.y);
+    }
+    if (e < nc) sC[e] = mk<T>((T)c.x, (T)c.y);
+    if (e < I.K) sSite[e] = st;
+    if (e < I.L + 2) sS[e] = (T)sqrt((double)e);
+  }
+}
+
 template <typename T, typename TI, typename TO, int STAGE, bool MFMA, bool HERM>
-__global__ __launch_bounds__(kThreads, PQA_PROJ_WAVES) void heom_proj_kernel(StageParams p) {
+__global__ __launch_bounds__(kThreads, (MFMA || !std::is_same<T, float>::value) ? PQA_PROJ_WAVES : PQA_VALU_WAVES)
+void heom_proj_kernel(StageParams p) {
   extern __shared__ __align__(16) char smem[];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: ADO indices stay scalar
   cpx<T>* sH = (cpx<T>*)smem;
-  cpx<T>* sC = sH + p.d2max;
+  cpx<T>* sC = sH + p.d2pad;
   T* sS = (T*)(sC + 5 * p.Kmax);
   int32_t* sSite = (int32_t*)(sS + p.Lmax + 2);
-  char* wbase = smem + proj_block_bytes(p.d2max, p.Kmax, p.Lmax, sizeof(T));
+  char* wbase = smem + proj_block_bytes(p.d2pad, p.Kmax, p.Lmax, sizeof(T));
   WaveLds<T> w;
   {
-    char* b = wbase + wv * proj_wave_bytes(p.d2max, p.rowbuf, p.colbuf, sizeof(T));
+    char* b = wbase + wv * proj_wave_bytes(p.d2pad, p.rowbuf, p.colbuf, sizeof(T));
     w.X = (cpx<T>*)b;
-    w.R = w.X + p.d2max;
+    w.R = w.X + p.d2pad;
     w.Cc = w.R + p.rowbuf;
     w.cl = w.Cc + p.colbuf;
     w.cr = w.cl + kMaxEnt;
@@ -472,38 +561,20 @@ __global__ __launch_bounds__(kThreads, PQA_PROJ_WAVES) void heom_proj_kernel(Sta
   int it0, itstep, itend;
   item_range(p, it0, itend, itstep);
   for (int it = it0; it < itend; it += itstep) {
-    const WorkItem wi = p.items[it];
-    const InstDev I = p.insts[wi.inst];
+    const WorkItem wi = ro_load(p.items + it);
+    const InstDev I = ro_load(p.insts + wi.inst);
     const int d = I.d, d2 = I.d2;
     // floor(idx / d) = (idx * dmag) >> 20 for idx < 2^11, d <= 32 (dmag*d - 2^20 < d < 2^20 / 2^11)
     const uint32_t dmag = ((1u << 20) + d - 1) / d;
+    // LDS row stride of the own matrix, H and the neighbour rows: d+1 (odd), so that column reads of
+    // the A operand (stride rs) and the rows of the row buffer fall on different banks
+    const int rs = d + 1;
     __syncthreads();
-    {
-      double fs = 0.0;
-      if (p.field) fs = p.field[6 * p.step + col];
-      const double2* H = p.mat + I.h_off;
-      const bool sd = (I.flags & F_SDIP) != 0;
-      for (int e = tid; e < d2; e += kThreads) {
-        double2 h = H[e];
-        if (sd) {
-          const double2 s2 = H[d2 + e];
-          h.x += s2.x * fs;
-          h.y += s2.y * fs;
-        }
-        sH[e] = mk<T>((T)h.x, (T)h.y);
-      }
-      for (int e = tid; e < 5 * I.K; e += kThreads) {
-        const double2 v = p.coefp[I.coefp_off + e];
-        sC[e] = mk<T>((T)v.x, (T)v.y);
-      }
-      for (int e = tid; e < I.K; e += kThreads) sSite[e] = p.site[I.site_off + e];
-      for (int e = tid; e < I.L + 2; e += kThreads) sS[e] = (T)sqrt((double)e);
-    }
+    proj_header<T>(p, I, col, dmag, sH, sC, sSite, sS);
     __syncthreads();
     // MFMA lane coordinates: lane = (gk, gi), gk = lane / 16, gi = lane % 16
     const int gi = lane & 15, gk = lane >> 4;
     const cpx_store<TI>* xi = xin + I.off;
-    const int rs = d + 1;  // row-buffer stride: odd, so rows do not start on the same LDS bank
     // Software pipeline over this wave's ADOs: the neighbour list (and, on the MFMA path, the own
     // matrix) of the next ADO are loaded while the current one is processed; they are issued after
     // the current ADO's neighbour loads, so waiting for those does not wait for the prefetch.
@@ -512,8 +583,8 @@ __global__ __launch_bounds__(kThreads, PQA_PROJ_WAVES) void heom_proj_kernel(Sta
     int2 en_n = make_int2(0, 0);
     cpx_store<TI> xt_n[4];
     auto prefetch = [&](int an) {
-      const int r0 = p.rowptr[I.rowptr_off + an];
-      nent_n = p.rowptr[I.rowptr_off + an + 1] - r0;
+      const int r0 = ro_int(p.rowptr, I.rowptr_off + an);
+      nent_n = ro_int(p.rowptr, I.rowptr_off + an + 1) - r0;
       en_n = make_int2(0, 0);
       if (lane < nent_n) en_n = p.ent[I.ent_off + r0 + lane];
       if (MFMA) {
@@ -534,24 +605,17 @@ __global__ __launch_bounds__(kThreads, PQA_PROJ_WAVES) void heom_proj_kernel(Sta
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int e = lane + 64 * q;
-          if (e < d2) w.X[e] = cvt<T>(mk(xt_n[q].x, xt_n[q].y));
+          if (e < d2) w.X[e + (int)(((uint32_t)e * dmag) >> 20)] = cvt<T>(mk(xt_n[q].x, xt_n[q].y));
         }
       } else {
-        for (int e = lane; e < d2; e += 64) w.X[e] = ldc<T>(xi, base + e);
+        for (int e = lane; e < d2; e += 64) w.X[e + (int)(((uint32_t)e * dmag) >> 20)] = ldc<T>(xi, base + e);
       }
-      // ---- neighbour list sorted by site (counting sort over ballots) with coefficients
+      // ---- neighbour list in site order, with coefficients
       const int k = en.y & 0xffff, nf = (en.y >> 16) & 0x7f;
       const bool plus = en.y < 0;
       const int my_site = lane < nent ? sSite[k] : 0x7fffffff;
-      int pos = 0, gsv = 0, run = 0;
-      const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-      for (int s2 = 0; s2 < d; ++s2) {
-        const uint64_t m = __ballot(my_site == s2);
-        if (lane == s2) gsv = run;
-        if (my_site == s2) pos = run + __popcll(m & below);
-        run += __popcll(m);
-      }
-      if (lane == d) gsv = run;
+      int gsv;
+      const int pos = site_order(my_site, d, lane, (I.flags & F_SITESORT) != 0, gsv);
       if (lane <= d) w.gs[lane] = gsv;
       cpx<T> dp = mk<T>(0, 0);
       if (lane < nent) {
@@ -591,19 +655,25 @@ __global__ __launch_bounds__(kThreads, PQA_PROJ_WAVES) void heom_proj_kernel(Sta
         }
       };
       issue(0, 0);
+      cpx<T> acc[4];
+      Pre pr[4];
+      if (MFMA && PQA_RK4_HOIST == 2) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (4 * gk + r < d && gi < d) pr[r] = rk4_prefetch<STAGE>(p, I.off + base + (4 * gk + r) * d + gi);
+      }
       if (a + kThreads / 64 < aend) prefetch(a + kThreads / 64);
       // ---- commutator and damping (while the neighbour loads are in flight)
-      cpx<T> acc[4];
       if (MFMA) {
         floatx4 aR = {0.f, 0.f, 0.f, 0.f}, aI = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kc = 0; kc < 4; ++kc) {
           const int kk = 4 * kc + gk;
           const bool ok = gi < d && kk < d;
-          const cpx<T> xb = ok ? w.X[kk * d + gi] : mk<T>(0, 0);  // B-layout X[kk][gi]
-          const cpx<T> xa = ok ? w.X[gi * d + kk] : mk<T>(0, 0);  // A-layout X[gi][kk]
-          const cpx<T> ha = ok ? sH[gi * d + kk] : mk<T>(0, 0);   // A-layout H[gi][kk]
-          const cpx<T> hb = ok ? sH[kk * d + gi] : mk<T>(0, 0);   // B-layout H[kk][gi]
+          const cpx<T> xb = ok ? w.X[kk * rs + gi] : mk<T>(0, 0);  // B-layout X[kk][gi]
+          const cpx<T> xa = ok ? w.X[gi * rs + kk] : mk<T>(0, 0);  // A-layout X[gi][kk]
+          const cpx<T> ha = ok ? sH[gi * rs + kk] : mk<T>(0, 0);   // A-layout H[gi][kk]
+          const cpx<T> hb = ok ? sH[kk * rs + gi] : mk<T>(0, 0);   // B-layout H[kk][gi]
           const float xbr = (float)xb.x, xbi = (float)xb.y, xar = (float)xa.x, xai = (float)xa.y;
           const float har = (float)ha.x, hai = (float)ha.y, hbr = (float)hb.x, hbi = (float)hb.y;
           // + H X
@@ -621,7 +691,7 @@ __global__ __launch_bounds__(kThreads, PQA_PROJ_WAVES) void heom_proj_kernel(Sta
         for (int r = 0; r < 4; ++r) {
           const int i = 4 * gk + r;
           acc[r] = mul_mi(mk<T>((T)aR[r], (T)aI[r]));
-          if (i < d && gi < d) cmsub(acc[r], damp, w.X[i * d + gi]);
+          if (i < d && gi < d) cmsub(acc[r], damp, w.X[i * rs + gi]);
         }
       }
       stash(0, 0);
@@ -630,6 +700,13 @@ __global__ __launch_bounds__(kThreads, PQA_PROJ_WAVES) void heom_proj_kernel(Sta
           issue(pass, q0);
           stash(pass, q0);
         }
+      // RK4 operands of the lane's four elements, requested together (one round trip instead of
+      // one per element); the gather below overlaps their latency
+      if (MFMA && PQA_RK4_HOIST == 1) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (4 * gk + r < d && gi < d) pr[r] = rk4_prefetch<STAGE>(p, I.off + base + (4 * gk + r) * d + gi);
+      }
       wave_sync();
       // ---- gather the neighbour terms from the row buffer, RK4 update
       const bool save_ado = STAGE == 3 && a == 0 && p.traj && p.save_slot >= 0;
@@ -649,31 +726,45 @@ __global__ __launch_bounds__(kThreads, PQA_PROJ_WAVES) void heom_proj_kernel(Sta
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int i = 4 * gk + r;
-            if (i < d) {
-              // row term, site i: element (i, gi) <- cL_t * R_t[gi]
+            // row term, site i: element (i, gi) <- cL_t * R_t[gi]
+            if (i < d)
               for (int t = w.gs[i]; t < w.gs[i + 1]; ++t) cmac(acc[r], w.cl[t], w.R[t * rs + gi]);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = 4 * gk + r;
+            if (i < d) {
               const int e = i * d + gi;
-              const Pre pr = rk4_prefetch<STAGE>(p, I.off + base + e);
-              rk4_store<TO, STAGE>(p, I.off + base + e, (double)acc[r].x, (double)acc[r].y, pr, xout, save_ado,
+              if (!PQA_RK4_HOIST) pr[r] = rk4_prefetch<STAGE>(p, I.off + base + e);
+              rk4_store<TO, STAGE>(p, I.off + base + e, (double)acc[r].x, (double)acc[r].y, pr[r], xout, save_ado,
                                    tbase + e);
             }
           }
         }
       } else {
-        for (int e = lane; e < d2; e += 64) {
-          const int i = e / d, j = e - (e / d) * d;
-          const Pre pr = rk4_prefetch<STAGE>(p, I.off + base + e);
-          cpx<T> c = mk<T>(0, 0);
-          for (int kk = 0; kk < d; ++kk) {
-            cmac(c, sH[i * d + kk], w.X[kk * d + j]);
-            cmsub(c, w.X[i * d + kk], sH[kk * d + j]);
+        // four elements per lane at a time: their RK4 operands are requested together
+        for (int e0 = 0; e0 < d2; e0 += 256) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (e0 + lane + 64 * q < d2) pr[q] = rk4_prefetch<STAGE>(p, I.off + base + e0 + lane + 64 * q);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int e = e0 + lane + 64 * q;
+            if (e < d2) {
+              const int i = (int)(((uint32_t)e * dmag) >> 20), j = e - i * d;
+              cpx<T> c = mk<T>(0, 0);
+              for (int kk = 0; kk < d; ++kk) {
+                cmac(c, sH[i * rs + kk], w.X[kk * rs + j]);
+                cmsub(c, w.X[i * rs + kk], sH[kk * rs + j]);
+              }
+              cpx<T> o = mul_mi(c);
+              cmsub(o, damp, w.X[i * rs + j]);
+              for (int t = w.gs[i]; t < w.gs[i + 1]; ++t) cmac(o, w.cl[t], w.R[t * rs + j]);
+              for (int t = w.gs[j]; t < w.gs[j + 1]; ++t)
+                cmac(o, w.cr[t], HERM ? (i != j ? conjg(w.R[t * rs + i]) : w.R[t * rs + i]) : w.Cc[t * rs + i]);
+              rk4_store<TO, STAGE>(p, I.off + base + e, (double)o.x, (double)o.y, pr[q], xout, save_ado, tbase + e);
+            }
           }
-          cpx<T> o = mul_mi(c);
-          cmsub(o, damp, w.X[e]);
-          for (int t = w.gs[i]; t < w.gs[i + 1]; ++t) cmac(o, w.cl[t], w.R[t * rs + j]);
-          for (int t = w.gs[j]; t < w.gs[j + 1]; ++t)
-            cmac(o, w.cr[t], HERM ? (i != j ? conjg(w.R[t * rs + i]) : w.R[t * rs + i]) : w.Cc[t * rs + i]);
-          rk4_store<TO, STAGE>(p, I.off + base + e, (double)o.x, (double)o.y, pr, xout, save_ado, tbase + e);
         }
       }
       wave_sync();
@@ -693,8 +784,8 @@ __global__ __launch_bounds__(kThreads, PQA_PROJ_WAVES) void heom_proj_kernel(Sta
 #define PQA_PIPE_WAVES 2
 #endif
 
-__host__ __device__ inline size_t pipe_wave_bytes(int d2max, int rowbuf) {
-  size_t b = 2 * (8 * ((size_t)d2max + rowbuf) + 2 * 8 * kMaxEnt + 2 * 4 * kMaxEnt + 4 * 36);
+__host__ __device__ inline size_t pipe_wave_bytes(int xsize, int rowbuf) {
+  size_t b = 2 * (8 * ((size_t)xsize + rowbuf) + 2 * 8 * kMaxEnt + 2 * 4 * kMaxEnt + 4 * 36);
   return (b + 15) & ~size_t(15);
 }
 
@@ -703,12 +794,13 @@ __global__ __launch_bounds__(kThreads, PQA_PIPE_WAVES) void heom_pipe_kernel(Sta
   using T = float;
   using TI = typename std::conditional<STAGE == 0, double, float>::type;
   extern __shared__ __align__(16) char smem[];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: ADO indices stay scalar
   cpx<T>* sH = (cpx<T>*)smem;
-  cpx<T>* sC = sH + p.d2max;
+  cpx<T>* sC = sH + p.d2pad;
   T* sS = (T*)(sC + 5 * p.Kmax);
   int32_t* sSite = (int32_t*)(sS + p.Lmax + 2);
-  char* wb = smem + proj_block_bytes(p.d2max, p.Kmax, p.Lmax, sizeof(T)) + wv * pipe_wave_bytes(p.d2max, p.rowbuf);
+  char* wb = smem + proj_block_bytes(p.d2pad, p.Kmax, p.Lmax, sizeof(T)) + wv * pipe_wave_bytes(p.d2pad, p.rowbuf);
   // two buffers of everything per wave
   cpx<T>* Xb[2];
   cpx<T>* Rb[2];
@@ -721,7 +813,7 @@ __global__ __launch_bounds__(kThreads, PQA_PIPE_WAVES) void heom_pipe_kernel(Sta
     char* q = wb;
     for (int u = 0; u < 2; ++u) {
       Xb[u] = (cpx<T>*)q;
-      q += 8 * (size_t)p.d2max;
+      q += 8 * (size_t)p.d2pad;
       Rb[u] = (cpx<T>*)q;
       q += 8 * (size_t)p.rowbuf;
       clb[u] = (cpx<T>*)q;
@@ -742,32 +834,12 @@ __global__ __launch_bounds__(kThreads, PQA_PIPE_WAVES) void heom_pipe_kernel(Sta
   int it0, itstep, itend;
   item_range(p, it0, itend, itstep);
   for (int it = it0; it < itend; it += itstep) {
-    const WorkItem wi = p.items[it];
-    const InstDev I = p.insts[wi.inst];
+    const WorkItem wi = ro_load(p.items + it);
+    const InstDev I = ro_load(p.insts + wi.inst);
     const int d = I.d, d2 = I.d2, rs = d + 1;
     const uint32_t dmag = ((1u << 20) + d - 1) / d;
     __syncthreads();
-    {
-      double fs = 0.0;
-      if (p.field) fs = p.field[6 * p.step + col];
-      const double2* H = p.mat + I.h_off;
-      const bool sd = (I.flags & F_SDIP) != 0;
-      for (int e = tid; e < d2; e += kThreads) {
-        double2 h = H[e];
-        if (sd) {
-          const double2 s2 = H[d2 + e];
-          h.x += s2.x * fs;
-          h.y += s2.y * fs;
-        }
-        sH[e] = mk<T>((T)h.x, (T)h.y);
-      }
-      for (int e = tid; e < 5 * I.K; e += kThreads) {
-        const double2 v = p.coefp[I.coefp_off + e];
-        sC[e] = mk<T>((T)v.x, (T)v.y);
-      }
-      for (int e = tid; e < I.K; e += kThreads) sSite[e] = p.site[I.site_off + e];
-      for (int e = tid; e < I.L + 2; e += kThreads) sS[e] = (T)sqrt((double)e);
-    }
+    proj_header<T>(p, I, col, dmag, sH, sC, sSite, sS);
     __syncthreads();
     const int gi = lane & 15, gk = lane >> 4;
     const cpx_store<TI>* xi = xin + I.off;
@@ -781,8 +853,8 @@ __global__ __launch_bounds__(kThreads, PQA_PIPE_WAVES) void heom_pipe_kernel(Sta
     cpx_store<TI> v[kChunk];
     int nval_n = 0;  // number of row values in flight for the next ADO
     auto fetch_list = [&](int an) {
-      const int r0 = p.rowptr[I.rowptr_off + an];
-      nent_n = p.rowptr[I.rowptr_off + an + 1] - r0;
+      const int r0 = ro_int(p.rowptr, I.rowptr_off + an);
+      nent_n = ro_int(p.rowptr, I.rowptr_off + an + 1) - r0;
       en_n = make_int2(0, 0);
       if (lane < nent_n) en_n = p.ent[I.ent_off + r0 + lane];
 #pragma unroll
@@ -796,21 +868,14 @@ __global__ __launch_bounds__(kThreads, PQA_PIPE_WAVES) void heom_pipe_kernel(Sta
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int e = lane + 64 * q;
-        if (e < d2) Xb[u][e] = cvt<T>(mk(xt_n[q].x, xt_n[q].y));
+        if (e < d2) Xb[u][e + (int)(((uint32_t)e * dmag) >> 20)] = cvt<T>(mk(xt_n[q].x, xt_n[q].y));
       }
       const int nent = nent_n;
       const int k = en_n.y & 0xffff, nf = (en_n.y >> 16) & 0x7f;
       const bool plus = en_n.y < 0;
       const int my_site = lane < nent ? sSite[k] : 0x7fffffff;
-      int pos = 0, gsv = 0, run = 0;
-      const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-      for (int s2 = 0; s2 < d; ++s2) {
-        const uint64_t m = __ballot(my_site == s2);
-        if (lane == s2) gsv = run;
-        if (my_site == s2) pos = run + __popcll(m & below);
-        run += __popcll(m);
-      }
-      if (lane == d) gsv = run;
+      int gsv;
+      const int pos = site_order(my_site, d, lane, (I.flags & F_SITESORT) != 0, gsv);
       if (lane <= d) gsb[u][lane] = gsv;
       cpx<T> dp = mk<T>(0, 0);
       if (lane < nent) {
@@ -870,10 +935,10 @@ __global__ __launch_bounds__(kThreads, PQA_PIPE_WAVES) void heom_pipe_kernel(Sta
       for (int kc = 0; kc < 4; ++kc) {
         const int kk = 4 * kc + gk;
         const bool ok = gi < d && kk < d;
-        const cpx<T> xb = ok ? Xb[u][kk * d + gi] : mk<T>(0, 0);
-        const cpx<T> xa = ok ? Xb[u][gi * d + kk] : mk<T>(0, 0);
-        const cpx<T> ha = ok ? sH[gi * d + kk] : mk<T>(0, 0);
-        const cpx<T> hb = ok ? sH[kk * d + gi] : mk<T>(0, 0);
+        const cpx<T> xb = ok ? Xb[u][kk * rs + gi] : mk<T>(0, 0);
+        const cpx<T> xa = ok ? Xb[u][gi * rs + kk] : mk<T>(0, 0);
+        const cpx<T> ha = ok ? sH[gi * rs + kk] : mk<T>(0, 0);
+        const cpx<T> hb = ok ? sH[kk * rs + gi] : mk<T>(0, 0);
         aR = __builtin_amdgcn_mfma_f32_16x16x4f32(ha.x, xb.x, aR, 0, 0, 0);
         aR = __builtin_amdgcn_mfma_f32_16x16x4f32(-ha.y, xb.y, aR, 0, 0, 0);
         aI = __builtin_amdgcn_mfma_f32_16x16x4f32(ha.x, xb.y, aI, 0, 0, 0);
@@ -888,7 +953,7 @@ __global__ __launch_bounds__(kThreads, PQA_PIPE_WAVES) void heom_pipe_kernel(Sta
       for (int r = 0; r < 4; ++r) {
         const int i = 4 * gk + r;
         acc[r] = mul_mi(mk<T>(aR[r], aI[r]));
-        if (i < d && gi < d) cmsub(acc[r], damp, Xb[u][i * d + gi]);
+        if (i < d && gi < d) cmsub(acc[r], damp, Xb[u][i * rs + gi]);
       }
       // rows of ADO a arrive; then its RK4 operands are requested
       stash_rows(u, nval);
@@ -1139,6 +1204,7 @@ struct pyqed_heom_plan {
   std::vector<int> inst_K, inst_L;
   int64_t n_elems = 0, n_ados = 0, traj_stride = 0;
   int d2max = 1, qmax = 1, Kmax = 1, Lmax = 0;
+  int d2pad = 2;
   std::vector<WorkItem> items;  // streamed items by class (generic, projector/VALU, projector/MFMA), then resident
   int cls_begin[4] = {0, 0, 0, 0};  // class c occupies [cls_begin[c], cls_begin[c+1])
   int n_items_stream = 0;
@@ -1163,6 +1229,11 @@ struct pyqed_heom_plan {
   int blocks_per_cu = 8;
   int variant = 0;  // 0: site-projector kernel; 1: pipelined kernel for the MFMA class (Hermitian, FP32)
   int rowbuf = 1, colbuf = 0;  // projector path: max entries*d over instances (columns: non-Hermitian)
+  // LDS dimensions of the projector kernels per item class (1: VALU, 2: MFMA commutator), so that
+  // a class of small instances is not sized by the largest
+  struct ProjDims {
+    int d2pad = 2, rowbuf = 1, colbuf = 0, Kmax = 1, Lmax = 0;
+  } pd[3];
   bool hermitian = false;
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -1187,13 +1258,14 @@ size_t stream_lds_bytes(const pyqed_heom_plan* P, size_t tsize) {
   return h + x;
 }
 
-size_t proj_lds_bytes(const pyqed_heom_plan* P, size_t tsize) {
-  return proj_block_bytes(P->d2max, P->Kmax, P->Lmax, tsize) +
-         (kThreads / 64) * proj_wave_bytes(P->d2max, P->rowbuf, P->colbuf, tsize);
+size_t proj_lds_bytes(const pyqed_heom_plan* P, int c, size_t tsize) {
+  const auto& q = P->pd[c];
+  return proj_block_bytes(q.d2pad, q.Kmax, q.Lmax, tsize) + (kThreads / 64) * proj_wave_bytes(q.d2pad, q.rowbuf, q.colbuf, tsize);
 }
 
-size_t pipe_lds_bytes(const pyqed_heom_plan* P) {
-  return proj_block_bytes(P->d2max, P->Kmax, P->Lmax, sizeof(float)) + (kThreads / 64) * pipe_wave_bytes(P->d2max, P->rowbuf);
+size_t pipe_lds_bytes(const pyqed_heom_plan* P, int c) {
+  const auto& q = P->pd[c];
+  return proj_block_bytes(q.d2pad, q.Kmax, q.Lmax, sizeof(float)) + (kThreads / 64) * pipe_wave_bytes(q.d2pad, q.rowbuf);
 }
 
 size_t resident_lds_bytes(const pyqed_heom_plan* P, int64_t n_el, size_t tsize) {
@@ -1415,7 +1487,13 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
       if (proj && s.K + std::min(s.K, s.L) <= kMaxEnt) {
         I.path = (s.d >= 12 && s.d <= 16) ? 2 : 1;
         I.site_off = (int)sites.size();
-        for (int k = 0; k < s.K; ++k) sites.push_back(psite[s.mode[k]]);
+        bool sorted_sites = true;
+        for (int k = 0; k < s.K; ++k) {
+          sites.push_back(psite[s.mode[k]]);
+          if (k > 0 && sites.back() < sites[sites.size() - 2]) sorted_sites = false;
+        }
+        // neighbour entries are listed by k, so a nondecreasing site map lists them in site order
+        if (sorted_sites) I.flags |= F_SITESORT;
         I.coefp_off = (int)coefp.size();
         for (int k = 0; k < s.K; ++k) {
           const std::complex<double> wgt = pw[s.mode[k]];
@@ -1430,6 +1508,12 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
         const int nb = (s.K + std::min(s.K, s.L)) * (s.d + 1);
         P->rowbuf = std::max(P->rowbuf, nb);
         if (!(flags & PYQED_PLAN_HERMITIAN)) P->colbuf = std::max(P->colbuf, nb);
+        auto& q = P->pd[I.path];
+        q.d2pad = std::max(q.d2pad, I.d2 + I.d);
+        q.rowbuf = std::max(q.rowbuf, nb);
+        if (!(flags & PYQED_PLAN_HERMITIAN)) q.colbuf = std::max(q.colbuf, nb);
+        q.Kmax = std::max(q.Kmax, s.K);
+        q.Lmax = std::max(q.Lmax, s.L);
       }
     }
     if (mat.size() > (size_t)INT32_MAX || coef.size() > (size_t)INT32_MAX || coefp.size() > (size_t)INT32_MAX) {
@@ -1444,6 +1528,7 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     toff += I.d2;
     ados += t.nmax;
     P->d2max = std::max(P->d2max, I.d2);
+    P->d2pad = std::max(P->d2pad, I.d2 + I.d);
     P->qmax = std::max(P->qmax, dense ? 1 : s.ncoup * s.d);
     P->Kmax = std::max(P->Kmax, s.K);
     P->Lmax = std::max(P->Lmax, s.L);
@@ -1634,6 +1719,7 @@ StageParams base_params(const pyqed_heom_plan* P) {
   p.coefp = P->d_coefp;
   p.traj_stride = P->traj_stride;
   p.d2max = P->d2max;
+  p.d2pad = P->d2pad;
   p.qmax = P->qmax;
   p.Kmax = P->Kmax;
   p.Lmax = P->Lmax;
@@ -1670,11 +1756,19 @@ int stream_grid(const pyqed_heom_plan* P, int n_items) {
 // One RK4 stage over every streamed item class: generic kernel, projector kernel with VALU
 // commutator, projector kernel with MFMA commutator (FP32 only; FP64 uses VALU).
 template <typename T, int S>
-int launch_stage(const pyqed_heom_plan* P, StageParams p, hipStream_t st) {
+int launch_stage(const pyqed_heom_plan* P, const StageParams& p0, hipStream_t st) {
   using TI = typename std::conditional<S == 0, double, T>::type;
   for (int c = 0; c < 3; ++c) {
     const int n = P->cls_begin[c + 1] - P->cls_begin[c];
     if (n == 0) continue;
+    StageParams p = p0;
+    if (c > 0) {  // projector kernels: LDS dimensions of this class
+      p.d2pad = P->pd[c].d2pad;
+      p.rowbuf = P->pd[c].rowbuf;
+      p.colbuf = P->pd[c].colbuf;
+      p.Kmax = P->pd[c].Kmax;
+      p.Lmax = P->pd[c].Lmax;
+    }
     p.item_begin = P->cls_begin[c];
     p.item_end = P->cls_begin[c + 1];
     p.use_xcd = 1;
@@ -1683,23 +1777,24 @@ int launch_stage(const pyqed_heom_plan* P, StageParams p, hipStream_t st) {
     int rc;
     // the projector kernel keeps neighbour rows (and columns) in LDS; when they do not fit, the
     // generic kernel (which any instance can use) takes the class
-    if (c == 0 || proj_lds_bytes(P, sizeof(T)) > (size_t)P->max_lds) {
+    const size_t plds = c == 0 ? 0 : proj_lds_bytes(P, c, sizeof(T));
+    if (c == 0 || plds > (size_t)P->max_lds) {
       rc = launch(heom_stage_kernel<T, TI, T, S>, grid, stream_lds_bytes(P, sizeof(T)), st, p);
     } else if (P->hermitian && c == 2 && std::is_same<T, float>::value && P->variant == 1 &&
-               pipe_lds_bytes(P) <= (size_t)P->max_lds) {
-      rc = launch(heom_pipe_kernel<S>, grid, pipe_lds_bytes(P), st, p);
+               pipe_lds_bytes(P, c) <= (size_t)P->max_lds) {
+      rc = launch(heom_pipe_kernel<S>, grid, pipe_lds_bytes(P, c), st, p);
     } else if (P->hermitian) {
       if (c == 2 && std::is_same<T, float>::value)
-        rc = launch(heom_proj_kernel<T, TI, T, S, std::is_same<T, float>::value, true>, grid, proj_lds_bytes(P, sizeof(T)),
+        rc = launch(heom_proj_kernel<T, TI, T, S, std::is_same<T, float>::value, true>, grid, plds,
                     st, p);
       else
-        rc = launch(heom_proj_kernel<T, TI, T, S, false, true>, grid, proj_lds_bytes(P, sizeof(T)), st, p);
+        rc = launch(heom_proj_kernel<T, TI, T, S, false, true>, grid, plds, st, p);
     } else {
       if (c == 2 && std::is_same<T, float>::value)
         rc = launch(heom_proj_kernel<T, TI, T, S, std::is_same<T, float>::value, false>, grid,
-                    proj_lds_bytes(P, sizeof(T)), st, p);
+                    plds, st, p);
       else
-        rc = launch(heom_proj_kernel<T, TI, T, S, false, false>, grid, proj_lds_bytes(P, sizeof(T)), st, p);
+        rc = launch(heom_proj_kernel<T, TI, T, S, false, false>, grid, plds, st, p);
     }
     if (rc) return rc;
   }

@@ -87,6 +87,33 @@ def test_hermitian_mode_random_hermitian_state(d, L, prec):
         assert err <= tol
 
 
+@pytest.mark.parametrize("d", [9, 14])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_unsorted_site_map(d, prec):
+    """Exponentials whose sites are not in increasing order of k: the projector kernels sort each
+    neighbour list by site (the in-order case only counts the site groups)."""
+    from pyqed_amd.heom.engine import HeomEngine
+    inst = _instance(d, 2, seed=d + 31, weight=0.8)
+    etal, etar, etaa, expn, mode = inst["bath"]
+    perm = np.random.default_rng(d).permutation(d)
+    inst["bath"] = (etal, etar, etaa, expn, perm[mode])
+    assert np.any(np.diff(perm[mode]) < 0)
+    rng = np.random.default_rng(d + 2)
+    nmax = int(HeomEngine([inst], resident=False).nmax[0])
+    A = (rng.normal(size=(nmax, d, d)) + 1j * rng.normal(size=(nmax, d, d))) * 0.1
+    state = A + np.conj(np.transpose(A, (0, 2, 1)))
+    ref = _oracle_step(inst, state, 0.05)
+    tol = 1e-12 if prec == "fp64" else 1e-5
+    for herm in (True, False):
+        eng = HeomEngine([inst], resident=False, hermitian=herm)
+        assert eng.classes()["items"][0] == 0
+        eng.set_state([state])
+        eng.advance(1, 0.05, prec)
+        err = np.abs(eng.hierarchy(0) - ref).max() / np.abs(ref).max()
+        print(f"d={d} {prec} hermitian={herm} err={err:.2e}")
+        assert err <= tol
+
+
 def test_hermitian_flag_rejected_for_complex_exponents():
     from pyqed_amd import _lib
     from pyqed_amd.heom.engine import HeomEngine
