@@ -12,8 +12,10 @@ Printed JSON line (rank 0):
   error_vs_fp64  max-abs population / coherence difference of every
                instance's reduced density matrix after warmup+steps steps
                against that FP64 run
-  roofline     stage kernels: algorithmic HBM bytes (208 B per complex element
-               per RK4 step, mixed) / measured time per step, vs 8 TB/s
+  roofline     the dominant stage kernel (the kernel-class launch with the
+               longest average time, timed per launch with HIP events on its
+               stream): algorithmic HBM bytes per launch / average launch time,
+               vs 8 TB/s; `step` gives the same for all stage kernels of a step
   cpu_baseline numpy FP64 restatement (the oracle), one thread, on a stratified
                sample of the same instances
 
@@ -32,8 +34,20 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ADO-updates/s, FP32-batched vs FP64 plain HEOM, 1 GPU, ≤1e-6 pop. error"
 HBM_PEAK_GBS = 8000.0
-BYTES_MIXED = 208  # per complex element per RK4 step (stage traffic of y, ynew, X)
-BYTES_FP64 = 256
+# Algorithmic HBM bytes of one stage launch over a class of F stage-buffer elements (full matrices,
+# complex of ts bytes) and S RK4-state elements (complex double; S = F, or the upper triangles of a
+# packed Hermitian plan):  ts * F * NX[stage] + 16 * S * NS[stage].
+#   stage 0: read y, write ynew, X (packed: also read the full-matrix copy of y)
+#   stage 1, 2: read X, y, ynew; write ynew, X'
+#   stage 3: read X, ynew; write y (packed: also write the full-matrix copy of y)
+# Unpacked mixed precision: 208 B per element per RK4 step; FP64: 256 B.
+NX = {False: (1, 2, 2, 1), True: (2, 2, 2, 2)}
+NS = (2, 3, 3, 2)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "heom_proj_stage1_exciton4096_traffic.json")
+
+
+def stage_bytes(stage, ts, F, S, packed):
+    return ts * F * NX[packed][stage] + 16 * S * NS[stage]
 
 
 def instances(workload, n_inst):
@@ -70,13 +84,14 @@ def max_errs(a, b):
 
 
 def fp64_plain(insts, dt, warmup, steps, torch):
-    """Plain FP64 kernel, one instance per plan, no LDS residency: time `steps` steps after `warmup`."""
+    """Plain FP64 kernel, one instance per plan, full-matrix state, no LDS residency: time `steps` steps
+    after `warmup`."""
     from pyqed_amd.heom.engine import HeomEngine
     total = 0.0
     finals = []
     ados = 0
     for inst in insts:
-        eng = HeomEngine([inst], resident=False)
+        eng = HeomEngine([inst], resident=False, packed=False)
         eng.advance(warmup, dt, "fp64")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -117,6 +132,30 @@ def cpu_baseline(insts, dt, budget_s=15.0):
                 sample="oracle.heom.run (numpy FP64, vectorised over ADOs, 1 thread), 1 RK4 step of one "
                        f"instance per (d, depth) stratum: {len(done)} strata ({done[0]} .. {done[-1]}), "
                        f"{ados} ADO-updates in {t_used:.1f} s")
+
+
+def kernel_name(stage, cls, precision, eng):
+    """Template name of the kernel a (stage, class) launch runs (as a kernel trace lists it)."""
+    T = "float" if precision == "mixed" else "double"
+    TI = "double" if (stage == 0 and not eng.packed) else T
+    if cls == 0:
+        return f"heom_stage_kernel<{T}, {TI}, {T}, {stage}>"
+    mfma = "true" if (cls == 2 and precision == "mixed") else "false"
+    herm = "true" if eng.hermitian else "false"
+    return f"heom_proj_kernel<{T}, {TI}, {T}, {stage}, {mfma}, {herm}>"
+
+
+def measured_traffic(name, workload, packed):
+    """HBM bytes per launch of `name` (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the committed
+    counter summary of the same workload and state layout, or None when there is none."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f)
+    if t.get("workload") != workload or bool(t.get("packed", False)) != bool(packed):
+        return None
+    k = t["per_dispatch"].get(name)
+    return None if k is None else k["fetch_bytes"] + k["write_bytes"]
 
 
 def stream_probe(torch):
@@ -161,6 +200,10 @@ def main():
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c5"])
     ap.add_argument("--n-inst", type=int, default=4096)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--state", default="auto", choices=["auto", "full"],
+                    help="RK4 state layout: auto = upper triangles for Hermitian hierarchies, full = full matrices")
+    ap.add_argument("--no-launch-timing", action="store_true",
+                    help="no per-launch HIP events (roofline from the step time only)")
     ap.add_argument("--no-fp64-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", type=int, default=0, metavar="CHUNK",
@@ -193,7 +236,7 @@ def main():
     insts, dt, desc = instances(args.workload, args.n_inst)
     print(f"[bench] workload {args.workload}: {len(insts)} instance(s); building plan", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    eng = HeomEngine(insts)
+    eng = HeomEngine(insts, packed=False if args.state == "full" else None)
     torch.cuda.synchronize()
     t_plan = time.perf_counter() - t0
     st = eng.stats()
@@ -205,7 +248,10 @@ def main():
     if args.graph:
         eng.advance_graph(args.graph, dt, args.precision, chunk=args.graph)  # capture (+ one replay) untimed
         torch.cuda.synchronize()
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    timing = not args.graph and not args.no_launch_timing
+    if timing:
+        eng.set_timing(True)  # event pair around every kernel-class launch, on the launch stream
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     evs[0].record()
@@ -214,22 +260,42 @@ def main():
             eng.advance_graph(min(args.graph, args.steps - s), dt, args.precision, chunk=args.graph)
         evs[-1].record()
     else:
-        for s in range(args.steps):
-            eng.advance(1, dt, args.precision)
-            evs[s + 1].record()
+        # one call for all timed steps, as DEOMSolver.run makes (a packed plan rebuilds the stage-0
+        # input from y once per call)
+        eng.advance(args.steps, dt, args.precision)
+        evs[-1].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    if args.graph:
-        step_ms = [evs[0].elapsed_time(evs[-1]) / args.steps]
-    else:
-        step_ms = [evs[s].elapsed_time(evs[s + 1]) for s in range(args.steps)]
+    step_ms = [evs[0].elapsed_time(evs[-1]) / args.steps]
     tiling = eng.tiling()
     value = eng.n_ados * args.steps / wall
     print(f"[bench] timed {args.steps} steps in {wall:.3f}s", file=sys.stderr, flush=True)
-    bpe = BYTES_MIXED if args.precision == "mixed" else BYTES_FP64
+    ts = 8 if args.precision == "mixed" else 16
+    F = eng.classes()["elems"]
+    S = eng.class_state()
+    step_bytes = sum(stage_bytes(s, ts, F[c], S[c], eng.packed) for s in range(4) for c in range(3))
     t_step = float(np.mean(step_ms)) / 1e3
-    achieved = bpe * st["stream_elems"] / t_step / 1e9
+    roofline = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
+                "step": {"bytes": step_bytes, "ms": t_step * 1e3, "achieved": step_bytes / t_step / 1e9,
+                         "frac": step_bytes / t_step / 1e9 / HBM_PEAK_GBS}}
+    if timing:
+        ms, launches = eng.read_timing()
+        eng.set_timing(False)
+        avg = {k: ms[k] / launches[k] for k in range(12) if launches[k]}
+        k = max(avg, key=avg.get)
+        s_dom, c_dom = divmod(k, 3)
+        b = stage_bytes(s_dom, ts, F[c_dom], S[c_dom], eng.packed)
+        name = kernel_name(s_dom, c_dom, args.precision, eng)
+        roofline.update({"kernel": name, "stage": s_dom, "class": c_dom, "launch_ms": avg[k],
+                         "bytes_per_launch": b, "achieved": b / (avg[k] / 1e3) / 1e9,
+                         "frac": b / (avg[k] / 1e3) / 1e9 / HBM_PEAK_GBS,
+                         "launch_ms_all": {f"s{kk // 3}c{kk % 3}": v for kk, v in sorted(avg.items())}})
+        roofline["traffic"] = measured_traffic(name, desc, eng.packed)
+    else:
+        roofline.update({"kernel": "all stage kernels of a step", "achieved": roofline["step"]["achieved"],
+                         "frac": roofline["step"]["frac"]})
     mixed_final = eng.reduced_states()
+    packed = eng.packed
     eng.close()
     del eng
     torch.cuda.empty_cache()
@@ -241,11 +307,9 @@ def main():
         "dtype": "f32 rhs / f64 state" if args.precision == "mixed" else "f64",
         "data": "synthetic (seeded numpy.random.default_rng; closed-form Drude + Pade baths)",
         "config": {"workload": desc, "instances": len(insts), "precision": args.precision},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "heom_stage_kernel (4 RK4-stage launches per step, streamed instances)",
-                     "bytes_per_step": bpe * st["stream_elems"], "step_ms_events": float(np.mean(step_ms))},
+        "roofline": roofline,
         "plan_build_s": t_plan, "plan": st, "tiling": tiling, "graph_chunk": args.graph or None,
+        "state_layout": "packed Hermitian (upper triangles)" if packed else "full matrices",
     }
     from math import comb
     out["config"]["ados"] = int(sum(comb(len(i["bath"][4]) + i["L"], i["L"]) for i in insts))

@@ -45,6 +45,12 @@ extern "C" {
                                     couplings, real expn and etar = conj(etal) every ADO then stays
                                     Hermitian, and the kernels read column terms as conjugated rows.
                                     Rejected (PYQED_EINVAL) when the coefficients do not allow it. */
+#define PYQED_PLAN_PACKED 16     /* with PYQED_PLAN_HERMITIAN: the RK4 state y (and ynew in the
+                                    workspace) holds only the upper triangle of every ADO, row by row,
+                                    d(d+1)/2 complex elements per ADO; plan_query's n_elems and
+                                    plan_layout's state offsets refer to that layout.  The stage inputs
+                                    stay full matrices.  No counterpart in the reference (which stores
+                                    full matrices); it halves the FP64 state traffic. */
 
 /* Thread-local description of the last error (never NULL). */
 const char* pyqed_last_error(void);
@@ -116,8 +122,10 @@ int pyqed_heom_tables_build(int32_t K, int32_t L, int64_t* keys, int32_t* plus, 
  * dt, starting at global step index `step0` (time step0*dt).
  * Replaces the time loop of DEOMSolver.run and rk4() (deom.py:725-766,
  * 1072-1114).
- *   y          device, [n_elems] complex double: the packed hierarchy state,
- *              updated in place (ADO 0 of each instance is the reduced rho).
+ *   y          device, [n_elems] complex double: the hierarchy state, instance
+ *              after instance, ADO after ADO (full d x d matrices, or upper
+ *              triangles with PYQED_PLAN_PACKED), updated in place (ADO 0 of
+ *              each instance is the reduced rho).
  *   workspace  device, >= ws_bytes for `precision` (plan_query); scratch.
  *   field      device [(step0+nsteps)*6] double or NULL: for step s,
  *              field[6s+0..2] = f_s(t), f_s(t+dt/2), f_s(t+dt) and
@@ -133,8 +141,10 @@ int pyqed_heom_run(pyqed_heom_plan* plan, int32_t precision, void* y, void* work
 
 /* Evaluate the right-hand side once: out = RHS(x) at time index `step`
  * stage column `col` (0: t, 1: t+dt/2, 2: t+dt) in FP64 (device pointers,
- * [n_elems] complex double).  Replaces rem_cal (deom.py:667-673); used to
- * check the operator against the dense generator. */
+ * full matrices of every ADO: sum over instances of nmax*d*d complex double,
+ * instance after instance, also for PYQED_PLAN_PACKED plans).  Replaces
+ * rem_cal (deom.py:667-673); used to check the operator against the dense
+ * generator. */
 int pyqed_heom_rhs(pyqed_heom_plan* plan, const void* x, void* out, const double* field, int64_t step,
                    int32_t col, void* stream);
 
@@ -156,6 +166,18 @@ int pyqed_heom_plan_set_variant(pyqed_heom_plan* plan, int32_t variant);
  * commutator, 2 site-projector with MFMA commutator): items and elements
  * (host arrays [3], either may be NULL). */
 int pyqed_heom_plan_classes(const pyqed_heom_plan* plan, int64_t* items, int64_t* elems);
+/* RK4-state elements (size in y) of the streamed work per kernel class (host array [3]): equal to
+ * the element counts of plan_classes, or the packed counts with PYQED_PLAN_PACKED.  No counterpart
+ * in the reference; used for the benchmark's byte model. */
+int pyqed_heom_plan_class_state(const pyqed_heom_plan* plan, int64_t* state_elems);
+/* Per-launch timing of the streamed stage kernels with HIP events on the run's stream (for the
+ * benchmark's roofline; no counterpart in the reference).  set_timing(1) records an event pair
+ * around every kernel-class launch of later pyqed_heom_run calls (launches captured into a graph
+ * are not timed); read_timing waits for the recorded launches and returns, for RK4 stage s (0..3)
+ * and class c (0..2) at index 3*s + c, the summed milliseconds and the number of launches (host
+ * arrays [12]), then clears the record.  set_timing(0) stops recording. */
+int pyqed_heom_plan_set_timing(pyqed_heom_plan* plan, int32_t enable);
+int pyqed_heom_plan_read_timing(pyqed_heom_plan* plan, double* ms, int64_t* launches);
 
 /* ------------------------------------------------------------------ Liouville-space RK4
  * Batched propagation of vec(rho) (row-major, length N = d*d) under

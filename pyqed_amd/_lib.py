@@ -18,12 +18,15 @@ PLAN_NO_RESIDENT = 1
 PLAN_DENSE_Q = 2
 PLAN_GENERIC = 4
 PLAN_HERMITIAN = 8
+PLAN_PACKED = 16
 
 SYMBOLS = [
     "pyqed_last_error", "pyqed_version", "pyqed_device_count",
     "pyqed_heom_plan_create", "pyqed_heom_plan_destroy", "pyqed_heom_plan_query", "pyqed_heom_plan_layout",
     "pyqed_heom_tables_export", "pyqed_heom_tables_size", "pyqed_heom_tables_build", "pyqed_heom_run", "pyqed_heom_rhs", "pyqed_heom_plan_stats",
-    "pyqed_heom_plan_classes", "pyqed_heom_plan_set_tiling", "pyqed_heom_plan_get_tiling", "pyqed_heom_plan_set_variant", "pyqed_liouville_run", "pyqed_liouville_adjoint",
+    "pyqed_heom_plan_classes", "pyqed_heom_plan_set_tiling", "pyqed_heom_plan_get_tiling", "pyqed_heom_plan_set_variant",
+    "pyqed_heom_plan_class_state", "pyqed_heom_plan_set_timing", "pyqed_heom_plan_read_timing",
+    "pyqed_liouville_run", "pyqed_liouville_adjoint",
 ]
 
 
@@ -74,6 +77,9 @@ def lib():
     L.pyqed_heom_plan_set_tiling.argtypes = [vp, i64, i32]
     L.pyqed_heom_plan_get_tiling.argtypes = [vp, C.POINTER(i64), C.POINTER(i32)]
     L.pyqed_heom_plan_set_variant.argtypes = [vp, i32]
+    L.pyqed_heom_plan_class_state.argtypes = [vp, vp]
+    L.pyqed_heom_plan_set_timing.argtypes = [vp, i32]
+    L.pyqed_heom_plan_read_timing.argtypes = [vp, vp, vp]
     L.pyqed_liouville_run.argtypes = [i32, i32, vp, i32, vp, vp, i64, dbl, i32, vp, vp, vp, i64, vp]
     L.pyqed_liouville_adjoint.argtypes = [i32, i32, vp, i32, vp, vp, i64, dbl, i32, vp, vp, vp, vp, vp, i64, vp]
     for name in SYMBOLS:

@@ -61,6 +61,8 @@ struct InstDev {
   int32_t path;          // 0 generic, 1 projector/VALU commutator, 2 projector/MFMA commutator
   int32_t coefp_off;     // coefficients scaled by the projector weights
   int32_t site_off;      // site of every exponential, [K]
+  int32_t dp;            // RK4 state elements per ADO: d*d, or d(d+1)/2 in packed Hermitian mode
+  int64_t poff;          // offset of ADO 0 in the RK4 state (y, ynew)
 };
 
 struct WorkItem {
@@ -286,63 +288,114 @@ __device__ __forceinline__ cpx<T> element_rhs(const StageParams& p, const InstDe
   return r;
 }
 
+// Address of element (i, j) of ADO a in the RK4 state (y, ynew).  The state holds either the full
+// matrix of every ADO (d*d elements from I.off) or, in packed Hermitian mode, its upper triangle row
+// by row (d(d+1)/2 elements from I.poff).  A lower element (i > j) of a packed ADO reads the
+// conjugate of (j, i) and writes no state; the stage inputs X are always full matrices.
+struct StateRef {
+  int64_t g;
+  bool lower;
+};
+
+template <bool PACK>
+__device__ __forceinline__ StateRef state_ref(const InstDev& I, int64_t a, int i, int j) {
+  StateRef r;
+  if (!PACK) {
+    r.g = I.off + a * I.d2 + (i * I.d + j);
+    r.lower = false;
+  } else {
+    r.lower = i > j;
+    const int u = r.lower ? j : i, v = r.lower ? i : j;
+    r.g = I.poff + a * I.dp + ((u * (2 * I.d - u - 1)) >> 1) + v;  // row u starts at u(2d-u+1)/2
+  }
+  return r;
+}
+
+__device__ __forceinline__ double2 ld_state(const double2* q, const StateRef& s) {
+  double2 v = q[s.g];
+  if (s.lower) v.y = -v.y;
+  return v;
+}
+
 // Values an element's RK4 update reads, prefetched ahead of the right-hand side.
 struct Pre {
   double2 y, an;
 };
 
 template <int STAGE>
-__device__ __forceinline__ Pre rk4_prefetch(const StageParams& p, int64_t g) {
+__device__ __forceinline__ Pre rk4_prefetch(const StageParams& p, const StateRef& s) {
   Pre v;
-  v.y = (STAGE <= 2) ? p.y[g] : make_double2(0, 0);
-  v.an = (STAGE >= 1 && STAGE <= 3) ? p.ynew[g] : make_double2(0, 0);
+  v.y = (STAGE <= 2) ? ld_state(p.y, s) : make_double2(0, 0);
+  v.an = (STAGE >= 1 && STAGE <= 3) ? ld_state(p.ynew, s) : make_double2(0, 0);
   return v;
 }
 
-template <typename TO, int STAGE>
-__device__ __forceinline__ void rk4_store(const StageParams& p, int64_t g, double rx, double ry, const Pre& v,
-                                          cpx_store<TO>* xout, bool save, int64_t tpos) {
+// RK4 stage update of one element (i, j) from its right-hand side (rx, ry): gx is its index in the
+// stage buffers (full matrices) and gxT that of (j, i), s its RK4 state address.  In packed mode
+// stage 3 also writes the new state to xout (the stage-0 input of the next step) as the element
+// and its conjugate at (j, i), from the upper element only, so that this copy is exactly the one
+// the unpack kernel makes from y.  `save` marks ADO 0 of an instance on a trajectory step; tpos is
+// the element's trajectory slot and tposT that of (j, i).
+template <typename TO, int STAGE, bool PACK>
+__device__ __forceinline__ void rk4_store(const StageParams& p, int64_t gx, int64_t gxT, const StateRef& s, double rx,
+                                          double ry, const Pre& v, cpx_store<TO>* xout, bool save, int64_t tpos,
+                                          int64_t tposT) {
   const double dt = p.dt;
   if (STAGE == 4) {
-    ((double2*)p.xout)[g] = make_double2(rx, ry);
+    ((double2*)p.xout)[gx] = make_double2(rx, ry);
   } else if (STAGE == 0) {
-    p.ynew[g] = make_double2(v.y.x + rx * (dt / 6), v.y.y + ry * (dt / 6));
-    st(xout + g, mk<TO>((TO)(v.y.x + rx * (dt / 2)), (TO)(v.y.y + ry * (dt / 2))));
+    if (!s.lower) p.ynew[s.g] = make_double2(v.y.x + rx * (dt / 6), v.y.y + ry * (dt / 6));
+    st(xout + gx, mk<TO>((TO)(v.y.x + rx * (dt / 2)), (TO)(v.y.y + ry * (dt / 2))));
   } else if (STAGE == 1 || STAGE == 2) {
-    p.ynew[g] = make_double2(v.an.x + rx * (dt / 3), v.an.y + ry * (dt / 3));
+    if (!s.lower) p.ynew[s.g] = make_double2(v.an.x + rx * (dt / 3), v.an.y + ry * (dt / 3));
     const double h = STAGE == 1 ? dt / 2 : dt;
-    st(xout + g, mk<TO>((TO)(v.y.x + rx * h), (TO)(v.y.y + ry * h)));
+    st(xout + gx, mk<TO>((TO)(v.y.x + rx * h), (TO)(v.y.y + ry * h)));
   } else {
     const double2 yn = make_double2(v.an.x + rx * (dt / 6), v.an.y + ry * (dt / 6));
-    p.y[g] = yn;
-    if (save) p.traj[tpos] = yn;
+    if (!s.lower) {
+      p.y[s.g] = yn;
+      // packed (Hermitian) mode: (j, i) is written as the conjugate of (i, j), except on the diagonal
+      if (save) {
+        p.traj[tpos] = yn;
+        if (PACK && tposT != tpos) p.traj[tposT] = make_double2(yn.x, -yn.y);
+      }
+      if (PACK && xout) {
+        st(xout + gx, mk<TO>((TO)yn.x, (TO)yn.y));
+        if (gxT != gx) st(xout + gxT, mk<TO>((TO)yn.x, (TO)(-yn.y)));
+      }
+    }
   }
 }
 
-// RK4 stage update of element g (absolute index) from its right-hand side (rx, ry).
-// `save` marks ADO 0 of an instance on a trajectory step; tpos is its slot index.
-template <typename TO, int STAGE>
-__device__ __forceinline__ void rk4_apply(const StageParams& p, int64_t g, double rx, double ry,
-                                          cpx_store<TO>* xout, bool save, int64_t tpos) {
+// Packed (Hermitian) mode, upper element (i <= j) at state index g: writes ynew (stages 0-2) or y
+// (stage 3) and returns the element of the stage output (X' for stages 0-2, the new state for
+// stage 3, whose full-matrix copy is the next step's stage-0 input).  The lower element (j, i) of
+// the output is its conjugate.
+template <int STAGE>
+__device__ __forceinline__ double2 rk4_upper(const StageParams& p, int64_t g, double rx, double ry, const Pre& v) {
   const double dt = p.dt;
-  if (STAGE == 4) {
-    ((double2*)p.xout)[g] = make_double2(rx, ry);
-  } else if (STAGE == 0) {
-    const double2 yv = p.y[g];
-    p.ynew[g] = make_double2(yv.x + rx * (dt / 6), yv.y + ry * (dt / 6));
-    st(xout + g, mk<TO>((TO)(yv.x + rx * (dt / 2)), (TO)(yv.y + ry * (dt / 2))));
+  if (STAGE == 0) {
+    p.ynew[g] = make_double2(v.y.x + rx * (dt / 6), v.y.y + ry * (dt / 6));
+    return make_double2(v.y.x + rx * (dt / 2), v.y.y + ry * (dt / 2));
   } else if (STAGE == 1 || STAGE == 2) {
-    const double2 yv = p.y[g];
-    const double2 an = p.ynew[g];
-    p.ynew[g] = make_double2(an.x + rx * (dt / 3), an.y + ry * (dt / 3));
+    p.ynew[g] = make_double2(v.an.x + rx * (dt / 3), v.an.y + ry * (dt / 3));
     const double h = STAGE == 1 ? dt / 2 : dt;
-    st(xout + g, mk<TO>((TO)(yv.x + rx * h), (TO)(yv.y + ry * h)));
+    return make_double2(v.y.x + rx * h, v.y.y + ry * h);
   } else {
-    const double2 an = p.ynew[g];
-    const double2 yn = make_double2(an.x + rx * (dt / 6), an.y + ry * (dt / 6));
+    const double2 yn = make_double2(v.an.x + rx * (dt / 6), v.an.y + ry * (dt / 6));
     p.y[g] = yn;
-    if (save) p.traj[tpos] = yn;
+    return yn;
   }
+}
+
+// Packed index of upper element (i, j), i <= j, within its ADO (rows of the upper triangle in order).
+__device__ __forceinline__ int packed_index(int i, int j, int d) { return ((i * (2 * d - i - 1)) >> 1) + j; }
+
+template <typename TO, int STAGE, bool PACK>
+__device__ __forceinline__ void rk4_apply(const StageParams& p, int64_t gx, int64_t gxT, const StateRef& s, double rx,
+                                          double ry, cpx_store<TO>* xout, bool save, int64_t tpos, int64_t tposT) {
+  const Pre v = rk4_prefetch<STAGE>(p, s);
+  rk4_store<TO, STAGE, PACK>(p, gx, gxT, s, rx, ry, v, xout, save, tpos, tposT);
 }
 
 __device__ __forceinline__ void item_range(const StageParams& p, int& it0, int& itend, int& itstep) {
@@ -362,7 +415,7 @@ __device__ __forceinline__ void item_range(const StageParams& p, int& it0, int& 
 
 // ------------------------------------------------------------------ streamed stage kernel
 // STAGE 0..3: RK4 stage; STAGE 4: out = RHS(xin) only (FP64 operator check).
-template <typename T, typename TI, typename TO, int STAGE>
+template <typename T, typename TI, typename TO, int STAGE, bool PACK>
 __global__ __launch_bounds__(kThreads) void heom_stage_kernel(StageParams p) {
   extern __shared__ __align__(16) char smem[];
   const Lds<T> l = carve<T>(smem, p);
@@ -399,8 +452,9 @@ __global__ __launch_bounds__(kThreads) void heom_stage_kernel(StageParams p) {
           const cpx<T> r = element_rhs<T>(p, I, l, own, xi, a, i, j, fc);
           const int64_t g = I.off + (int64_t)a * d2 + e;
           const bool save = STAGE == 3 && a == 0 && p.traj && p.save_slot >= 0;
-          rk4_apply<TO, STAGE>(p, g, (double)r.x, (double)r.y, xout, save,
-                               p.save_slot * p.traj_stride + I.traj_off + e);
+          const int64_t tb = p.save_slot * p.traj_stride + I.traj_off;
+          rk4_apply<TO, STAGE, PACK>(p, g, I.off + (int64_t)a * d2 + j * d + i, state_ref<PACK>(I, a, i, j), (double)r.x,
+                               (double)r.y, xout, save, tb + e, tb + j * d + i);
         }
       }
     }
@@ -491,14 +545,20 @@ struct WaveLds {
   int32_t* nb;  // per entry: neighbour ADO
   int32_t* st;  // per entry: site
   int32_t* gs;  // site-group starts, d+1
+  cpx<T>* Tt;   // packed mode: the upper elements of the stage output, d*d (row stride d+1)
 };
 
-__host__ __device__ inline size_t proj_wave_bytes(int xsize, int rowbuf, int colbuf, size_t ts) {
+// Per-wave LDS of the projector kernel; `tile` adds the stage-output tile of packed mode.
+__host__ __device__ inline size_t proj_wave_bytes(int xsize, int rowbuf, int colbuf, size_t ts, bool tile) {
   size_t b = 2 * ts * ((size_t)xsize + rowbuf + colbuf) + 2 * 2 * ts * kMaxEnt + 2 * 4 * kMaxEnt + 4 * 36;
-  return (b + 15) & ~size_t(15);
+  b = (b + 15) & ~size_t(15);
+  return tile ? b + ((2 * ts * (size_t)xsize + 15) & ~size_t(15)) : b;
 }
-__host__ __device__ inline size_t proj_block_bytes(int xsize, int Kmax, int Lmax, size_t ts) {
+// Per-block LDS of the projector kernel: H, coefficients, sqrt table, sites and, in packed mode, the
+// (i, j) of every packed index (at most xsize / 2 of them).
+__host__ __device__ inline size_t proj_block_bytes(int xsize, int Kmax, int Lmax, size_t ts, bool pack) {
   size_t b = 2 * ts * (size_t)xsize + 2 * ts * 5 * (size_t)Kmax + ts * (size_t)(Lmax + 2) + 4 * (size_t)Kmax;
+  if (pack) b += 4 * ((size_t)xsize / 2 + 1);
   return (b + 15) & ~size_t(15);
 }
 
@@ -507,7 +567,7 @@ __host__ __device__ inline size_t proj_block_bytes(int xsize, int Kmax, int Lmax
 // issues all of its loads before the first LDS store waits for them.
 template <typename T>
 __device__ __forceinline__ void proj_header(const StageParams& p, const InstDev& I, int col, uint32_t dmag,
-                                            cpx<T>* sH, cpx<T>* sC, int32_t* sSite, T* sS) {
+                                            cpx<T>* sH, cpx<T>* sC, int32_t* sSite, T* sS, int32_t* sIJ) {
   const double fs = p.field ? ro_load(p.field + 6 * p.step + col) : 0.0;
   const double2* H = p.mat + I.h_off;
   const bool sd = (I.flags & F_SDIP) != 0;
@@ -530,9 +590,17 @@ __device__ __forceinline__ void proj_header(const StageParams& p, const InstDev&
     if (e < I.K) sSite[e] = st;
     if (e < I.L + 2) sS[e] = (T)sqrt((double)e);
   }
+  if (sIJ) {
+    const int d = I.d, np = (d * (d + 1)) >> 1;
+    for (int q = threadIdx.x; q < np; q += kThreads) {
+      int i = 0;
+      while (((i + 1) * (2 * d - i)) / 2 <= q) ++i;  // row i starts at i(2d-i+1)/2
+      sIJ[q] = (i << 8) | (q - ((i * (2 * d - i + 1)) >> 1) + i);
+    }
+  }
 }
 
-template <typename T, typename TI, typename TO, int STAGE, bool MFMA, bool HERM>
+template <typename T, typename TI, typename TO, int STAGE, bool MFMA, bool HERM, bool PACK>
 __global__ __launch_bounds__(kThreads, (MFMA || !std::is_same<T, float>::value) ? PQA_PROJ_WAVES : PQA_VALU_WAVES)
 void heom_proj_kernel(StageParams p) {
   extern __shared__ __align__(16) char smem[];
@@ -542,10 +610,12 @@ void heom_proj_kernel(StageParams p) {
   cpx<T>* sC = sH + p.d2pad;
   T* sS = (T*)(sC + 5 * p.Kmax);
   int32_t* sSite = (int32_t*)(sS + p.Lmax + 2);
-  char* wbase = smem + proj_block_bytes(p.d2pad, p.Kmax, p.Lmax, sizeof(T));
+  int32_t* sIJ = sSite + p.Kmax;  // packed mode: (i << 8) | j of packed index p
+  char* wbase = smem + proj_block_bytes(p.d2pad, p.Kmax, p.Lmax, sizeof(T), PACK);
   WaveLds<T> w;
   {
-    char* b = wbase + wv * proj_wave_bytes(p.d2pad, p.rowbuf, p.colbuf, sizeof(T));
+    char* b = wbase + wv * proj_wave_bytes(p.d2pad, p.rowbuf, p.colbuf, sizeof(T), PACK);
+    w.Tt = (cpx<T>*)(b + proj_wave_bytes(p.d2pad, p.rowbuf, p.colbuf, sizeof(T), false));
     w.X = (cpx<T>*)b;
     w.R = w.X + p.d2pad;
     w.Cc = w.R + p.rowbuf;
@@ -570,7 +640,7 @@ void heom_proj_kernel(StageParams p) {
     // the A operand (stride rs) and the rows of the row buffer fall on different banks
     const int rs = d + 1;
     __syncthreads();
-    proj_header<T>(p, I, col, dmag, sH, sC, sSite, sS);
+    proj_header<T>(p, I, col, dmag, sH, sC, sSite, sS, PACK ? sIJ : nullptr);
     __syncthreads();
     // MFMA lane coordinates: lane = (gk, gi), gk = lane / 16, gi = lane % 16
     const int gi = lane & 15, gk = lane >> 4;
@@ -657,10 +727,10 @@ void heom_proj_kernel(StageParams p) {
       issue(0, 0);
       cpx<T> acc[4];
       Pre pr[4];
-      if (MFMA && PQA_RK4_HOIST == 2) {
+      if (MFMA && !PACK && PQA_RK4_HOIST == 2) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (4 * gk + r < d && gi < d) pr[r] = rk4_prefetch<STAGE>(p, I.off + base + (4 * gk + r) * d + gi);
+          if (4 * gk + r < d && gi < d) pr[r] = rk4_prefetch<STAGE>(p, state_ref<PACK>(I, a, 4 * gk + r, gi));
       }
       if (a + kThreads / 64 < aend) prefetch(a + kThreads / 64);
       // ---- commutator and damping (while the neighbour loads are in flight)
@@ -702,16 +772,102 @@ void heom_proj_kernel(StageParams p) {
         }
       // RK4 operands of the lane's four elements, requested together (one round trip instead of
       // one per element); the gather below overlaps their latency
-      if (MFMA && PQA_RK4_HOIST == 1) {
+      if (MFMA && PACK) {
+        // packed mode: RK4 operands of this lane's upper elements, packed indices lane + 64q (d <= 16:
+        // at most 136 of them), read in packed order
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const int pk = lane + 64 * q;
+          if (pk < ((d * (d + 1)) >> 1)) pr[q] = rk4_prefetch<STAGE>(p, StateRef{I.poff + (int64_t)a * I.dp + pk, false});
+        }
+      } else if (MFMA && PQA_RK4_HOIST == 1) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (4 * gk + r < d && gi < d) pr[r] = rk4_prefetch<STAGE>(p, I.off + base + (4 * gk + r) * d + gi);
+          if (4 * gk + r < d && gi < d) pr[r] = rk4_prefetch<STAGE>(p, state_ref<PACK>(I, a, 4 * gk + r, gi));
       }
       wave_sync();
       // ---- gather the neighbour terms from the row buffer, RK4 update
       const bool save_ado = STAGE == 3 && a == 0 && p.traj && p.save_slot >= 0;
       const int64_t tbase = p.save_slot * p.traj_stride + I.traj_off;
-      if (MFMA) {
+      if constexpr (PACK) {
+        // Packed (Hermitian) mode: lane takes the upper elements of packed index lane + 64q; their
+        // stage outputs and the conjugates at (j, i) go to the tile, which is then stored as full
+        // matrices.  Neighbour terms of upper element (i, j): row term of site i and column term of
+        // site j read as conjugated rows (the diagonal keeps the row value itself).
+        const int np = (d * (d + 1)) >> 1;
+        auto neighbours = [&](cpx<T>& o, int i, int j) {
+          for (int t = w.gs[i]; t < w.gs[i + 1]; ++t) cmac(o, w.cl[t], w.R[t * rs + j]);
+          for (int t = w.gs[j]; t < w.gs[j + 1]; ++t)
+            cmac(o, w.cr[t], i != j ? conjg(w.R[t * rs + i]) : w.R[t * rs + i]);
+        };
+        auto output = [&](int i, int j, const double2& o) {
+          w.Tt[i * rs + j] = mk<T>((T)o.x, (T)o.y);
+          if (i != j) w.Tt[j * rs + i] = mk<T>((T)o.x, (T)(-o.y));
+          if (save_ado) {
+            p.traj[tbase + i * d + j] = o;
+            if (i != j) p.traj[tbase + j * d + i] = make_double2(o.x, -o.y);
+          }
+        };
+        if (MFMA) {
+          // commutator and damping of every element (MFMA lane layout) through the tile
+          if (gi < d) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              if (4 * gk + r < d) w.Tt[(4 * gk + r) * rs + gi] = acc[r];
+          }
+          wave_sync();
+          cpx<T> au[3];
+          int ij[3];
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            const int pk = lane + 64 * q;
+            if (pk < np) {
+              ij[q] = sIJ[pk];
+              const int i = ij[q] >> 8, j = ij[q] & 255;
+              au[q] = w.Tt[i * rs + j];
+              neighbours(au[q], i, j);
+            }
+          }
+          wave_sync();  // every tile read is done: the tile now takes the stage output
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            const int pk = lane + 64 * q;
+            if (pk < np)
+              output(ij[q] >> 8, ij[q] & 255,
+                     rk4_upper<STAGE>(p, I.poff + (int64_t)a * I.dp + pk, (double)au[q].x, (double)au[q].y, pr[q]));
+          }
+        } else {
+          for (int p0 = 0; p0 < np; p0 += 256) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int pk = p0 + lane + 64 * q;
+              if (pk < np) pr[q] = rk4_prefetch<STAGE>(p, StateRef{I.poff + (int64_t)a * I.dp + pk, false});
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int pk = p0 + lane + 64 * q;
+              if (pk < np) {
+                const int ij = sIJ[pk], i = ij >> 8, j = ij & 255;
+                cpx<T> c = mk<T>(0, 0);
+                for (int kk = 0; kk < d; ++kk) {
+                  cmac(c, sH[i * rs + kk], w.X[kk * rs + j]);
+                  cmsub(c, w.X[i * rs + kk], sH[kk * rs + j]);
+                }
+                cpx<T> o = mul_mi(c);
+                cmsub(o, damp, w.X[i * rs + j]);
+                neighbours(o, i, j);
+                output(i, j, rk4_upper<STAGE>(p, I.poff + (int64_t)a * I.dp + pk, (double)o.x, (double)o.y, pr[q]));
+              }
+            }
+          }
+        }
+        wave_sync();
+        for (int e = lane; e < d2; e += 64) {
+          const int i = (int)(((uint32_t)e * dmag) >> 20), j = e - i * d;
+          const cpx<T> u = w.Tt[i * rs + j];
+          st(xout + I.off + base + e, mk<TO>((TO)u.x, (TO)u.y));
+        }
+      } else if (MFMA) {
         if (gi < d) {
           // column term, site gi: element (4gk+r, gi) <- cR_t * conj(R_t[4gk+r])
           for (int t = w.gs[gi]; t < w.gs[gi + 1]; ++t) {
@@ -735,9 +891,10 @@ void heom_proj_kernel(StageParams p) {
             const int i = 4 * gk + r;
             if (i < d) {
               const int e = i * d + gi;
-              if (!PQA_RK4_HOIST) pr[r] = rk4_prefetch<STAGE>(p, I.off + base + e);
-              rk4_store<TO, STAGE>(p, I.off + base + e, (double)acc[r].x, (double)acc[r].y, pr[r], xout, save_ado,
-                                   tbase + e);
+              const StateRef sr = state_ref<PACK>(I, a, i, gi);
+              if (!PQA_RK4_HOIST) pr[r] = rk4_prefetch<STAGE>(p, sr);
+              rk4_store<TO, STAGE, PACK>(p, I.off + base + e, I.off + base + gi * d + i, sr, (double)acc[r].x,
+                                         (double)acc[r].y, pr[r], xout, save_ado, tbase + e, tbase + gi * d + i);
             }
           }
         }
@@ -745,8 +902,11 @@ void heom_proj_kernel(StageParams p) {
         // four elements per lane at a time: their RK4 operands are requested together
         for (int e0 = 0; e0 < d2; e0 += 256) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q)
-            if (e0 + lane + 64 * q < d2) pr[q] = rk4_prefetch<STAGE>(p, I.off + base + e0 + lane + 64 * q);
+          for (int q = 0; q < 4; ++q) {
+            const int e = e0 + lane + 64 * q;
+            const int i = (int)(((uint32_t)e * dmag) >> 20);
+            if (e < d2) pr[q] = rk4_prefetch<STAGE>(p, state_ref<PACK>(I, a, i, e - i * d));
+          }
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int e = e0 + lane + 64 * q;
@@ -762,7 +922,8 @@ void heom_proj_kernel(StageParams p) {
               for (int t = w.gs[i]; t < w.gs[i + 1]; ++t) cmac(o, w.cl[t], w.R[t * rs + j]);
               for (int t = w.gs[j]; t < w.gs[j + 1]; ++t)
                 cmac(o, w.cr[t], HERM ? (i != j ? conjg(w.R[t * rs + i]) : w.R[t * rs + i]) : w.Cc[t * rs + i]);
-              rk4_store<TO, STAGE>(p, I.off + base + e, (double)o.x, (double)o.y, pr[q], xout, save_ado, tbase + e);
+              rk4_store<TO, STAGE, PACK>(p, I.off + base + e, I.off + base + j * d + i, state_ref<PACK>(I, a, i, j),
+                                         (double)o.x, (double)o.y, pr[q], xout, save_ado, tbase + e, tbase + j * d + i);
             }
           }
         }
@@ -800,7 +961,7 @@ __global__ __launch_bounds__(kThreads, PQA_PIPE_WAVES) void heom_pipe_kernel(Sta
   cpx<T>* sC = sH + p.d2pad;
   T* sS = (T*)(sC + 5 * p.Kmax);
   int32_t* sSite = (int32_t*)(sS + p.Lmax + 2);
-  char* wb = smem + proj_block_bytes(p.d2pad, p.Kmax, p.Lmax, sizeof(T)) + wv * pipe_wave_bytes(p.d2pad, p.rowbuf);
+  char* wb = smem + proj_block_bytes(p.d2pad, p.Kmax, p.Lmax, sizeof(T), false) + wv * pipe_wave_bytes(p.d2pad, p.rowbuf);
   // two buffers of everything per wave
   cpx<T>* Xb[2];
   cpx<T>* Rb[2];
@@ -839,7 +1000,7 @@ __global__ __launch_bounds__(kThreads, PQA_PIPE_WAVES) void heom_pipe_kernel(Sta
     const int d = I.d, d2 = I.d2, rs = d + 1;
     const uint32_t dmag = ((1u << 20) + d - 1) / d;
     __syncthreads();
-    proj_header<T>(p, I, col, dmag, sH, sC, sSite, sS);
+    proj_header<T>(p, I, col, dmag, sH, sC, sSite, sS, nullptr);
     __syncthreads();
     const int gi = lane & 15, gk = lane >> 4;
     const cpx_store<TI>* xi = xin + I.off;
@@ -960,7 +1121,7 @@ __global__ __launch_bounds__(kThreads, PQA_PIPE_WAVES) void heom_pipe_kernel(Sta
       Pre pre[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        if (4 * gk + r < d && gi < d) pre[r] = rk4_prefetch<STAGE>(p, I.off + base + (4 * gk + r) * d + gi);
+        if (4 * gk + r < d && gi < d) pre[r] = rk4_prefetch<STAGE>(p, state_ref<false>(I, a, 4 * gk + r, gi));
       wave_sync();
       // next ADO: sort its list, request its rows, fetch the list after it
       cpx<T> damp_n = mk<T>(0, 0);
@@ -988,8 +1149,8 @@ __global__ __launch_bounds__(kThreads, PQA_PIPE_WAVES) void heom_pipe_kernel(Sta
           if (i < d) {
             for (int t = gsb[u][i]; t < gsb[u][i + 1]; ++t) cmac(acc[r], clb[u][t], Rb[u][t * rs + gi]);
             const int e = i * d + gi;
-            rk4_store<T, STAGE>(p, I.off + base + e, (double)acc[r].x, (double)acc[r].y, pre[r], xout, save_ado,
-                                tbase + e);
+            rk4_store<T, STAGE, false>(p, I.off + base + e, I.off + base + gi * d + i, state_ref<false>(I, a, i, gi),
+                                (double)acc[r].x, (double)acc[r].y, pre[r], xout, save_ado, tbase + e, tbase + gi * d + i);
           }
         }
       }
@@ -1004,7 +1165,7 @@ __global__ __launch_bounds__(kThreads, PQA_PIPE_WAVES) void heom_pipe_kernel(Sta
 // One workgroup owns one small instance for the whole run: y, ynew and both
 // stage buffers live in LDS, so HBM is touched only to load the initial state,
 // write trajectory samples and store the final state.
-template <typename T>
+template <typename T, bool PACK>
 __global__ __launch_bounds__(kThreads) void heom_resident_kernel(StageParams p) {
   extern __shared__ __align__(16) char smem[];
   const Lds<T> l = carve<T>(smem, p);
@@ -1017,7 +1178,8 @@ __global__ __launch_bounds__(kThreads) void heom_resident_kernel(StageParams p) 
   const int tid = threadIdx.x;
   const double dt = p.dt;
   for (int e = tid; e < n; e += kThreads) {
-    double2 v = p.y[I.off + e];
+    const int a = e / d2, r = e - a * d2, i = r / d;
+    const double2 v = ld_state(p.y, state_ref<PACK>(I, a, i, r - i * d));
     Y[e] = mk(v.x, v.y);
   }
   for (int64_t s = 0; s < p.nsteps; ++s) {
@@ -1060,6 +1222,15 @@ __global__ __launch_bounds__(kThreads) void heom_resident_kernel(StageParams p) 
       }
     }
     __syncthreads();
+    if (PACK) {
+      // packed (Hermitian) state: the lower triangle is the conjugate of the upper one after every
+      // step, as a reload of the packed y would make it
+      for (int e = tid; e < n; e += kThreads) {
+        const int a = e / d2, r = e - a * d2, i = r / d, j = r - i * d;
+        if (i > j) Y[e] = conjg(Y[a * d2 + j * d + i]);
+      }
+      __syncthreads();
+    }
     if (p.traj && ((s + 1) % p.save_every) == 0) {
       const int64_t slot = (s + 1) / p.save_every - 1;
       for (int e = tid; e < d2; e += kThreads) {
@@ -1068,7 +1239,11 @@ __global__ __launch_bounds__(kThreads) void heom_resident_kernel(StageParams p) 
     }
   }
   __syncthreads();
-  for (int e = tid; e < n; e += kThreads) p.y[I.off + e] = make_double2(Y[e].x, Y[e].y);
+  for (int e = tid; e < n; e += kThreads) {
+    const int a = e / d2, r = e - a * d2, i = r / d;
+    const StateRef sr = state_ref<PACK>(I, a, i, r - i * d);
+    if (!sr.lower) p.y[sr.g] = make_double2(Y[e].x, Y[e].y);
+  }
 }
 
 }  // namespace
@@ -1235,8 +1410,15 @@ struct pyqed_heom_plan {
     int d2pad = 2, rowbuf = 1, colbuf = 0, Kmax = 1, Lmax = 0;
   } pd[3];
   bool hermitian = false;
+  bool packed = false;     // y, ynew hold upper triangles (PYQED_PLAN_PACKED)
+  int64_t n_state = 0;     // RK4 state elements (size of y): n_elems, or the packed count
+  int64_t cls_state[3] = {0, 0, 0};  // RK4 state elements of the streamed items per class
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // per-launch timing (pyqed_heom_plan_set_timing): event pairs around every class launch
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_tag;  // stage*3 + class of pair k (events 2k, 2k+1)
 };
 
 namespace {
@@ -1252,6 +1434,11 @@ int upload(const std::vector<V>& h, V** d) {
   return PYQED_OK;
 }
 
+// Workspace of a run: ynew in the RK4 state layout (complex double), then the two stage buffers
+// (full matrices, complex of xsize bytes).
+int64_t ws_state_bytes(const pyqed_heom_plan* P) { return ((P->n_state * 16 + 255) / 256) * 256; }
+int64_t workspace_bytes(const pyqed_heom_plan* P, size_t xsize) { return ws_state_bytes(P) + 2 * P->n_elems * (int64_t)xsize; }
+
 size_t stream_lds_bytes(const pyqed_heom_plan* P, size_t tsize) {
   size_t h = lds_header_bytes(P->d2max, P->qmax, P->Kmax, P->Lmax, tsize);
   size_t x = 2 * tsize * (size_t)std::max(256, 4 * P->d2max);
@@ -1260,12 +1447,13 @@ size_t stream_lds_bytes(const pyqed_heom_plan* P, size_t tsize) {
 
 size_t proj_lds_bytes(const pyqed_heom_plan* P, int c, size_t tsize) {
   const auto& q = P->pd[c];
-  return proj_block_bytes(q.d2pad, q.Kmax, q.Lmax, tsize) + (kThreads / 64) * proj_wave_bytes(q.d2pad, q.rowbuf, q.colbuf, tsize);
+  return proj_block_bytes(q.d2pad, q.Kmax, q.Lmax, tsize, P->packed) +
+         (kThreads / 64) * proj_wave_bytes(q.d2pad, q.rowbuf, q.colbuf, tsize, P->packed);
 }
 
 size_t pipe_lds_bytes(const pyqed_heom_plan* P, int c) {
   const auto& q = P->pd[c];
-  return proj_block_bytes(q.d2pad, q.Kmax, q.Lmax, sizeof(float)) + (kThreads / 64) * pipe_wave_bytes(q.d2pad, q.rowbuf);
+  return proj_block_bytes(q.d2pad, q.Kmax, q.Lmax, sizeof(float), false) + (kThreads / 64) * pipe_wave_bytes(q.d2pad, q.rowbuf);
 }
 
 size_t resident_lds_bytes(const pyqed_heom_plan* P, int64_t n_el, size_t tsize) {
@@ -1294,6 +1482,7 @@ void destroy_plan(pyqed_heom_plan* P) {
   if (P->side) (void)hipStreamDestroy(P->side);
   if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
   if (P->ev_join) (void)hipEventDestroy(P->ev_join);
+  for (hipEvent_t e : P->ev_pool) (void)hipEventDestroy(e);
   delete P;
 }
 
@@ -1355,7 +1544,12 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
   P->n_inst = n_inst;
   P->flags = flags;
   P->hermitian = (flags & PYQED_PLAN_HERMITIAN) != 0;
+  P->packed = (flags & PYQED_PLAN_PACKED) != 0;
   const bool dense = (flags & PYQED_PLAN_DENSE_Q) != 0;
+  if (P->packed && !P->hermitian) {
+    destroy_plan(P);
+    return fail(PYQED_EINVAL, "plan_create: PYQED_PLAN_PACKED needs PYQED_PLAN_HERMITIAN");
+  }
   // ---- validate and build tables
   for (int i = 0; i < n_inst; ++i) {
     const pyqed_heom_instance& s = in[i];
@@ -1406,7 +1600,7 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
   }
   std::vector<double2> coef, mat, coefp;
   std::vector<int32_t> mode, sites;
-  int64_t off = 0, toff = 0, ados = 0;
+  int64_t off = 0, toff = 0, ados = 0, soff = 0;
   for (int i = 0; i < n_inst; ++i) {
     const pyqed_heom_instance& s = in[i];
     const Tables& t = *P->tabs.at({s.K, s.L});
@@ -1421,6 +1615,8 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     I.flags = (s.sdip ? F_SDIP : 0) | (dense ? (s.cmat ? F_CDIP : 0) : (s.cdiag ? F_CDIP : 0)) | (dense ? F_DENSE : 0);
     I.grp = group_size(I.d2);
     I.off = off;
+    I.dp = P->packed ? s.d * (s.d + 1) / 2 : I.d2;
+    I.poff = P->packed ? soff : off;
     I.traj_off = toff;
     I.rowptr_off = tab_off[{s.K, s.L}].first;
     I.ent_off = tab_off[{s.K, s.L}].second;
@@ -1525,6 +1721,7 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     P->inst_K.push_back(s.K);
     P->inst_L.push_back(s.L);
     off += (int64_t)t.nmax * I.d2;
+    soff += (int64_t)t.nmax * I.dp;
     toff += I.d2;
     ados += t.nmax;
     P->d2max = std::max(P->d2max, I.d2);
@@ -1534,6 +1731,7 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     P->Lmax = std::max(P->Lmax, s.L);
   }
   P->n_elems = off;
+  P->n_state = P->packed ? soff : off;
   P->n_ados = ados;
   P->traj_stride = toff;
   // ---- device properties
@@ -1560,6 +1758,8 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
   for (int i = 0; i < n_inst; ++i)
     if (is_res[i]) P->resident.push_back(i);
   retile(P, 0, 8);
+  for (int i = 0; i < n_inst; ++i)
+    if (!is_res[i]) P->cls_state[P->inst[i].path] += (int64_t)P->inst[i].nmax * P->inst[i].dp;
   if (stream_lds_bytes(P, sizeof(double)) > (size_t)P->max_lds) {
     destroy_plan(P);
     return fail(PYQED_EUNSUPPORTED, "instance too large for the streamed kernel's LDS budget");
@@ -1592,20 +1792,50 @@ extern "C" int pyqed_heom_plan_destroy(pyqed_heom_plan* plan) {
 extern "C" int pyqed_heom_plan_query(const pyqed_heom_plan* P, int64_t* n_elems, int64_t* n_ados,
                                      int64_t* ws64, int64_t* wsmix) {
   if (!P) return fail(PYQED_EINVAL, "plan_query: null plan");
-  if (n_elems) *n_elems = P->n_elems;
+  if (n_elems) *n_elems = P->n_state;
   if (n_ados) *n_ados = P->n_ados;
-  if (ws64) *ws64 = P->n_elems * 48;
-  if (wsmix) *wsmix = P->n_elems * 32;
+  if (ws64) *ws64 = workspace_bytes(P, sizeof(double2));
+  if (wsmix) *wsmix = workspace_bytes(P, sizeof(float2));
   return PYQED_OK;
 }
 
 extern "C" int pyqed_heom_plan_layout(const pyqed_heom_plan* P, int64_t* so, int64_t* nm, int64_t* to) {
   if (!P) return fail(PYQED_EINVAL, "plan_layout: null plan");
   for (int i = 0; i < P->n_inst; ++i) {
-    if (so) so[i] = P->inst[i].off;
+    if (so) so[i] = P->inst[i].poff;
     if (nm) nm[i] = P->inst[i].nmax;
     if (to) to[i] = P->inst[i].traj_off;
   }
+  return PYQED_OK;
+}
+
+extern "C" int pyqed_heom_plan_class_state(const pyqed_heom_plan* P, int64_t* state_elems) {
+  if (!P || !state_elems) return fail(PYQED_EINVAL, "plan_class_state: null argument");
+  for (int c = 0; c < 3; ++c) state_elems[c] = P->cls_state[c];
+  return PYQED_OK;
+}
+
+extern "C" int pyqed_heom_plan_set_timing(pyqed_heom_plan* P, int32_t enable) {
+  if (!P) return fail(PYQED_EINVAL, "plan_set_timing: null plan");
+  P->timing = enable != 0;
+  P->ev_tag.clear();
+  return PYQED_OK;
+}
+
+extern "C" int pyqed_heom_plan_read_timing(pyqed_heom_plan* P, double* ms, int64_t* launches) {
+  if (!P || !ms || !launches) return fail(PYQED_EINVAL, "plan_read_timing: null argument");
+  for (int k = 0; k < 12; ++k) {
+    ms[k] = 0.0;
+    launches[k] = 0;
+  }
+  for (size_t k = 0; k < P->ev_tag.size(); ++k) {
+    PQA_HIP(hipEventSynchronize(P->ev_pool[2 * k + 1]));
+    float t = 0.f;
+    PQA_HIP(hipEventElapsedTime(&t, P->ev_pool[2 * k], P->ev_pool[2 * k + 1]));
+    ms[P->ev_tag[k]] += t;
+    launches[P->ev_tag[k]] += 1;
+  }
+  P->ev_tag.clear();
   return PYQED_OK;
 }
 
@@ -1755,9 +1985,37 @@ int stream_grid(const pyqed_heom_plan* P, int n_items) {
 
 // One RK4 stage over every streamed item class: generic kernel, projector kernel with VALU
 // commutator, projector kernel with MFMA commutator (FP32 only; FP64 uses VALU).
-template <typename T, int S>
-int launch_stage(const pyqed_heom_plan* P, const StageParams& p0, hipStream_t st) {
-  using TI = typename std::conditional<S == 0, double, T>::type;
+// Event pair around one class launch when the plan records per-launch timing (never while the
+// stream is being captured into a graph).
+struct LaunchTimer {
+  pyqed_heom_plan* P;
+  hipStream_t st;
+  int k = -1;
+  LaunchTimer(pyqed_heom_plan* P_, hipStream_t st_, int tag) : P(P_), st(st_) {
+    if (!P->timing) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+    k = (int)P->ev_tag.size();
+    while ((int)P->ev_pool.size() < 2 * k + 2) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) {
+        k = -1;
+        return;
+      }
+      P->ev_pool.push_back(e);
+    }
+    P->ev_tag.push_back(tag);
+    (void)hipEventRecord(P->ev_pool[2 * k], st);
+  }
+  ~LaunchTimer() {
+    if (k >= 0) (void)hipEventRecord(P->ev_pool[2 * k + 1], st);
+  }
+};
+
+// One RK4 stage over every streamed item class.  TI is the stage-input precision: FP64 for stage 0
+// of an unpacked plan (it reads y), the stage precision otherwise.
+template <typename T, int S, typename TI, bool PACK>
+int launch_stage_ti(pyqed_heom_plan* P, const StageParams& p0, hipStream_t st) {
   for (int c = 0; c < 3; ++c) {
     const int n = P->cls_begin[c + 1] - P->cls_begin[c];
     if (n == 0) continue;
@@ -1778,27 +2036,56 @@ int launch_stage(const pyqed_heom_plan* P, const StageParams& p0, hipStream_t st
     // the projector kernel keeps neighbour rows (and columns) in LDS; when they do not fit, the
     // generic kernel (which any instance can use) takes the class
     const size_t plds = c == 0 ? 0 : proj_lds_bytes(P, c, sizeof(T));
+    LaunchTimer timer(P, st, 3 * S + c);
+    constexpr bool F32 = std::is_same<T, float>::value;
     if (c == 0 || plds > (size_t)P->max_lds) {
-      rc = launch(heom_stage_kernel<T, TI, T, S>, grid, stream_lds_bytes(P, sizeof(T)), st, p);
-    } else if (P->hermitian && c == 2 && std::is_same<T, float>::value && P->variant == 1 &&
-               pipe_lds_bytes(P, c) <= (size_t)P->max_lds) {
+      rc = launch(heom_stage_kernel<T, TI, T, S, PACK>, grid, stream_lds_bytes(P, sizeof(T)), st, p);
+    } else if constexpr (PACK) {  // packed plans are Hermitian
+      if (c == 2 && F32)
+        rc = launch(heom_proj_kernel<T, TI, T, S, F32, true, true>, grid, plds, st, p);
+      else
+        rc = launch(heom_proj_kernel<T, TI, T, S, false, true, true>, grid, plds, st, p);
+    } else if (P->hermitian && c == 2 && F32 && P->variant == 1 && pipe_lds_bytes(P, c) <= (size_t)P->max_lds) {
       rc = launch(heom_pipe_kernel<S>, grid, pipe_lds_bytes(P, c), st, p);
     } else if (P->hermitian) {
-      if (c == 2 && std::is_same<T, float>::value)
-        rc = launch(heom_proj_kernel<T, TI, T, S, std::is_same<T, float>::value, true>, grid, plds,
-                    st, p);
+      if (c == 2 && F32)
+        rc = launch(heom_proj_kernel<T, TI, T, S, F32, true, false>, grid, plds, st, p);
       else
-        rc = launch(heom_proj_kernel<T, TI, T, S, false, true>, grid, plds, st, p);
+        rc = launch(heom_proj_kernel<T, TI, T, S, false, true, false>, grid, plds, st, p);
     } else {
-      if (c == 2 && std::is_same<T, float>::value)
-        rc = launch(heom_proj_kernel<T, TI, T, S, std::is_same<T, float>::value, false>, grid,
-                    plds, st, p);
+      if (c == 2 && F32)
+        rc = launch(heom_proj_kernel<T, TI, T, S, F32, false, false>, grid, plds, st, p);
       else
-        rc = launch(heom_proj_kernel<T, TI, T, S, false, false>, grid, plds, st, p);
+        rc = launch(heom_proj_kernel<T, TI, T, S, false, false, false>, grid, plds, st, p);
     }
     if (rc) return rc;
   }
   return PYQED_OK;
+}
+
+template <typename T, int S>
+int launch_stage(pyqed_heom_plan* P, const StageParams& p0, hipStream_t st) {
+  if (P->packed) return launch_stage_ti<T, S, T, true>(P, p0, st);
+  // unpacked stage 0 reads y itself (FP64)
+  return launch_stage_ti<T, S, typename std::conditional<S == 0, double, T>::type, false>(P, p0, st);
+}
+
+// Packed mode: the stage-0 input of the first step, X = y as full matrices in the stage precision
+// (stage 3 writes it for every later step).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void heom_unpack_kernel(StageParams p) {
+  cpx_store<T>* xout = (cpx_store<T>*)p.xout;
+  for (int it = p.item_begin + blockIdx.x; it < p.item_end; it += gridDim.x) {
+    const WorkItem w = ro_load(p.items + it);
+    const InstDev I = ro_load(p.insts + w.inst);
+    const int d = I.d, d2 = I.d2;
+    const int n = w.nado * d2;
+    for (int q = threadIdx.x; q < n; q += kThreads) {
+      const int a = w.ado0 + q / d2, e = q - (q / d2) * d2, i = e / d;
+      const double2 v = ld_state(p.y, state_ref<true>(I, a, i, e - i * d));
+      st(xout + I.off + (int64_t)a * d2 + e, mk<T>((T)v.x, (T)v.y));
+    }
+  }
 }
 
 template <typename T>
@@ -1806,7 +2093,7 @@ int run_typed(pyqed_heom_plan* P, double2* y, char* ws, const double* field, int
               double dt, double2* traj, int64_t save_every, hipStream_t st) {
   const int64_t n = P->n_elems;
   double2* ynew = (double2*)ws;
-  cpx_store<T>* x0 = (cpx_store<T>*)(ws + 16 * n);
+  cpx_store<T>* x0 = (cpx_store<T>*)(ws + ws_state_bytes(P));
   cpx_store<T>* x1 = x0 + n;
   StageParams p = base_params(P);
   p.dt = dt;
@@ -1824,15 +2111,25 @@ int run_typed(pyqed_heom_plan* P, double2* y, char* ws, const double* field, int
     int64_t nres_max = 0;
     for (int i : P->resident) nres_max = std::max<int64_t>(nres_max, (int64_t)P->inst[i].nmax * P->inst[i].d2);
     const size_t lds = resident_lds_bytes(P, nres_max, sizeof(T));
-    int rc = launch(heom_resident_kernel<T>, (int)P->resident.size(), lds, P->side, r);
+    int rc = P->packed ? launch(heom_resident_kernel<T, true>, (int)P->resident.size(), lds, P->side, r)
+                       : launch(heom_resident_kernel<T, false>, (int)P->resident.size(), lds, P->side, r);
     if (rc) return rc;
     PQA_HIP(hipEventRecord(P->ev_join, P->side));
   }
   if (P->n_items_stream > 0) {
+    // packed mode: stage 0 reads x1, the full-matrix copy of y that stage 3 leaves there
+    if (P->packed) {
+      StageParams u = p;
+      u.item_begin = 0;
+      u.item_end = P->n_items_stream;
+      u.xout = x1;
+      const int rc = launch(heom_unpack_kernel<T>, std::max(1, std::min(P->n_items_stream, P->num_cu * 8)), 0, st, u);
+      if (rc) return rc;
+    }
     for (int64_t s = 0; s < nsteps; ++s) {
       p.step = step0 + s;
       int rc;
-      p.xin = y;
+      p.xin = P->packed ? (const void*)x1 : (const void*)y;
       p.xout = x0;
       if ((rc = launch_stage<T, 0>(P, p, st))) return rc;
       p.xin = x0;
@@ -1842,7 +2139,7 @@ int run_typed(pyqed_heom_plan* P, double2* y, char* ws, const double* field, int
       p.xout = x0;
       if ((rc = launch_stage<T, 2>(P, p, st))) return rc;
       p.xin = x0;
-      p.xout = nullptr;
+      p.xout = P->packed ? x1 : nullptr;
       p.save_slot = (traj && ((s + 1) % save_every) == 0) ? (s + 1) / save_every - 1 : -1;
       if ((rc = launch_stage<T, 3>(P, p, st))) return rc;
       p.save_slot = -1;
@@ -1860,7 +2157,7 @@ extern "C" int pyqed_heom_run(pyqed_heom_plan* P, int32_t precision, void* y, vo
   if (!P || !y || !workspace) return fail(PYQED_EINVAL, "heom_run: null plan, state or workspace");
   if (nsteps < 0 || step0 < 0) return fail(PYQED_EINVAL, "heom_run: negative step count");
   if (save_every <= 0) save_every = 1;
-  const int64_t need = precision == PYQED_PREC_FP64 ? P->n_elems * 48 : P->n_elems * 32;
+  const int64_t need = workspace_bytes(P, precision == PYQED_PREC_FP64 ? sizeof(double2) : sizeof(float2));
   if (precision != PYQED_PREC_FP64 && precision != PYQED_PREC_MIXED)
     return fail(PYQED_EINVAL, "heom_run: precision must be PYQED_PREC_FP64 or PYQED_PREC_MIXED");
   if (ws_bytes < need)
@@ -1889,5 +2186,5 @@ extern "C" int pyqed_heom_rhs(pyqed_heom_plan* P, const void* x, void* out, cons
   p.col_override = col;
   const int grid = std::max(1, std::min((int)P->items.size(), P->num_cu * 8));
   const size_t lds = stream_lds_bytes(P, sizeof(double));
-  return launch(heom_stage_kernel<double, double, double, 4>, grid, lds, (hipStream_t)stream, p);
+  return launch(heom_stage_kernel<double, double, double, 4, false>, grid, lds, (hipStream_t)stream, p);
 }

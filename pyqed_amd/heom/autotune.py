@@ -1,8 +1,8 @@
 """Autotuner for the HEOM work queue: chooses the work-item size (complex elements per item) and
 the persistent grid (blocks per CU) of the streamed stage kernels by timing RK4 steps.
 
-Results are cached per workload signature (instances' (d, K, L), Hermitian mode, precision, device
-name) in pyqed_amd/tuning.json; HeomEngine applies a cached entry before its first step.
+Results are cached per workload signature (instances' (d, K, L), Hermitian mode, state layout,
+precision, device name) in pyqed_amd/tuning.json; HeomEngine applies a cached entry before its first step.
 
     python -m pyqed_amd.heom.autotune --workload c3 [--out tuning.json]
 """
@@ -26,6 +26,8 @@ def signature(engine, precision):
     torch = engine.torch
     key = [torch.cuda.get_device_name(engine.device), precision, engine.hermitian, engine.dense,
            [(p["d"], len(p["mode"]), p["L"]) for p in engine.prep]]
+    if getattr(engine, "packed", False):
+        key.append("packed")  # packed Hermitian state: its own entry
     return hashlib.sha1(json.dumps(key).encode()).hexdigest()[:20]
 
 

@@ -89,10 +89,30 @@ def _hermitian_dynamics(p):
             and np.all(np.abs(etar - np.conj(etal)) <= 1e-14 * np.abs(etal)))
 
 
+def pack_upper(a):
+    """Upper triangles (diagonal included) of matrices [n, d, d], row by row, as one flat array: the
+    packed RK4-state layout, element (u, v), u <= v, of an ADO at u(2d-u-1)/2 + v."""
+    a = np.asarray(a, np.complex128)
+    d = a.shape[-1]
+    iu = np.triu_indices(d)
+    return a.reshape(-1, d, d)[:, iu[0], iu[1]].reshape(-1)
+
+
+def unpack_upper(v, d):
+    """Full Hermitian matrices [n, d, d] from packed upper triangles (inverse of pack_upper)."""
+    iu = np.triu_indices(d)
+    v = np.asarray(v, np.complex128).reshape(-1, len(iu[0]))
+    up = np.zeros((v.shape[0], d, d), np.complex128)
+    up[:, iu[0], iu[1]] = v
+    low = np.conj(np.transpose(up, (0, 2, 1)))
+    low[:, np.arange(d), np.arange(d)] = 0
+    return up + low
+
+
 class HeomEngine:
     """A plan over one or more HEOM instances plus its device state."""
 
-    def __init__(self, instances, *, resident=True, generic=False, hermitian=None):
+    def __init__(self, instances, *, resident=True, generic=False, hermitian=None, packed=None):
         self.torch = torch = _lib.require_gpu()
         self.L = _lib.lib()
         self.prep = [prepare_instance(i) for i in instances]
@@ -121,8 +141,11 @@ class HeomEngine:
         if hermitian is None:
             hermitian = not self.dense and all(_hermitian_dynamics(p) for p in self.prep)
         self.hermitian = bool(hermitian)
+        # Hermitian hierarchies keep only the upper triangle of every ADO in the FP64 state
+        self.packed = self.hermitian if packed is None else bool(packed)
         flags = ((0 if resident else _lib.PLAN_NO_RESIDENT) | (_lib.PLAN_DENSE_Q if self.dense else 0) |
-                 (_lib.PLAN_GENERIC if generic else 0) | (_lib.PLAN_HERMITIAN if self.hermitian else 0))
+                 (_lib.PLAN_GENERIC if generic else 0) | (_lib.PLAN_HERMITIAN if self.hermitian else 0) |
+                 (_lib.PLAN_PACKED if self.packed else 0))
         plan = C.c_void_p()
         _lib.check(self.L.pyqed_heom_plan_create(arr, self.n, flags, C.byref(plan)))
         self.plan = plan
@@ -147,15 +170,31 @@ class HeomEngine:
         self.reset()
 
     # ------------------------------------------------------------------ state
+    # ------------------------------------------------------------------ state layout
+    def state_size(self, k):
+        """RK4-state elements per ADO of instance k: d*d, or d(d+1)/2 when packed."""
+        d = self.prep[k]["d"]
+        return d * (d + 1) // 2 if self.packed else d * d
+
+    def _to_state(self, k, a):
+        """[n, d, d] matrices -> [n * state_size] state elements (upper triangles when packed)."""
+        d = self.prep[k]["d"]
+        a = np.asarray(a, np.complex128).reshape(-1, d, d)
+        return pack_upper(a) if self.packed else a.reshape(-1)
+
+    def _from_state(self, k, v):
+        """Inverse of _to_state: full [n, d, d] matrices (Hermitian completion when packed)."""
+        d = self.prep[k]["d"]
+        return unpack_upper(v, d) if self.packed else np.asarray(v).reshape(-1, d, d)
+
+    # ------------------------------------------------------------------ state
     def reset(self):
         """y <- every instance at rho0 in ADO 0, zero elsewhere; time index back to 0."""
         torch = self.torch
         self.y.zero_()
-        host = np.zeros(self.traj_stride, np.complex128)
-        for k, p in enumerate(self.prep):
-            host[self.traj_offset[k]:self.traj_offset[k] + p["d"] ** 2] = p["rho0"].reshape(-1)
-        idx = np.concatenate([self.state_offset[k] + np.arange(p["d"] ** 2) for k, p in enumerate(self.prep)])
-        self.y[torch.as_tensor(idx, device=self.device)] = torch.as_tensor(host, device=self.device)
+        vals = [self._to_state(k, p["rho0"]) for k, p in enumerate(self.prep)]
+        idx = np.concatenate([self.state_offset[k] + np.arange(v.size) for k, v in enumerate(vals)])
+        self.y[torch.as_tensor(idx, device=self.device)] = torch.as_tensor(np.concatenate(vals), device=self.device)
         self.step = 0
 
     def workspace(self, prec):
@@ -174,7 +213,8 @@ class HeomEngine:
             if U is not None:
                 a = U.conj().T @ a @ U
             o = int(self.state_offset[k])
-            self.y[o:o + a.size] = torch.as_tensor(a.reshape(-1), device=self.device)
+            v = self._to_state(k, a)
+            self.y[o:o + v.size] = torch.as_tensor(v, device=self.device)
 
     def tiling(self):
         t, b = C.c_int64(), C.c_int32()
@@ -186,6 +226,24 @@ class HeomEngine:
         el = np.zeros(3, np.int64)
         _lib.check(self.L.pyqed_heom_plan_classes(self.plan, it.ctypes.data, el.ctypes.data))
         return dict(items=it.tolist(), elems=el.tolist())
+
+    def class_state(self):
+        """RK4-state elements of the streamed work per kernel class (packed counts when packed)."""
+        st = np.zeros(3, np.int64)
+        _lib.check(self.L.pyqed_heom_plan_class_state(self.plan, st.ctypes.data))
+        return st.tolist()
+
+    def set_timing(self, enable):
+        """Record an HIP event pair around every kernel-class launch of later advance() calls."""
+        _lib.check(self.L.pyqed_heom_plan_set_timing(self.plan, 1 if enable else 0))
+
+    def read_timing(self):
+        """(ms[12], launches[12]) per RK4 stage s and kernel class c at index 3s + c, summed over the
+        launches recorded since set_timing / the last read."""
+        ms = np.zeros(12, np.float64)
+        n = np.zeros(12, np.int64)
+        _lib.check(self.L.pyqed_heom_plan_read_timing(self.plan, ms.ctypes.data, n.ctypes.data))
+        return ms.tolist(), n.tolist()
 
     def stats(self):
         s = np.zeros(4, np.int64)
@@ -279,23 +337,21 @@ class HeomEngine:
     def reduced_states(self):
         """Current rho (ADO 0) of every instance, host complex128, caller's basis."""
         out = []
-        for k, p in enumerate(self.prep):
-            d = p["d"]
+        for k in range(self.n):
             o = int(self.state_offset[k])
-            r = self.y[o:o + d * d].cpu().numpy().reshape(d, d)
+            r = self._from_state(k, self.y[o:o + self.state_size(k)].cpu().numpy())[0]
             out.append(self._unrotate(k, r))
         return out
 
     def hierarchy(self, k=0):
         """Full ADO state of instance k as host array [nmax, d, d] (caller's basis)."""
-        p = self.prep[k]
-        d = p["d"]
         o = int(self.state_offset[k])
-        a = self.y[o:o + int(self.nmax[k]) * d * d].cpu().numpy().reshape(-1, d, d)
+        a = self._from_state(k, self.y[o:o + int(self.nmax[k]) * self.state_size(k)].cpu().numpy())
         return self._unrotate(k, a)
 
     def rhs(self, x, field=None, step=0, col=0):
-        """FP64 right-hand side of a packed state x (device complex128 [n_elems])."""
+        """FP64 right-hand side of full hierarchy matrices x (device complex128, every ADO of every
+        instance as a full d x d matrix, instance after instance; also for packed engines)."""
         torch = self.torch
         out = torch.empty_like(x)
         _lib.check(self.L.pyqed_heom_rhs(self.plan, x.data_ptr(), out.data_ptr(),

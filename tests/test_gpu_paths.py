@@ -164,7 +164,7 @@ def test_pipelined_variant_bitwise_equal(stage_prec):
     insts = [_instance(d, L, seed=200 + d) for d in (12, 14, 16) for L in (2, 3)]
     outs = []
     for variant in (0, 1):
-        eng = HeomEngine(insts, resident=False)
+        eng = HeomEngine(insts, resident=False, packed=False)  # the pipelined kernel has no packed form
         eng.autotune_cache = False
         _lib.check(eng.L.pyqed_heom_plan_set_tiling(eng.plan, 1024, 4))
         _lib.check(eng.L.pyqed_heom_plan_set_variant(eng.plan, variant))
