@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # Unpacked mixed precision: 208 B per element per RK4 step; FP64: 256 B.
 NX = {False: (1, 2, 2, 1), True: (2, 2, 2, 2)}
 NS = (2, 3, 3, 2)
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "heom_proj_stage1_exciton4096_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "heom_proj_kernels_exciton4096_traffic.json")
 
 
 def stage_bytes(stage, ts, F, S, packed):
@@ -138,11 +138,12 @@ def kernel_name(stage, cls, precision, eng):
     """Template name of the kernel a (stage, class) launch runs (as a kernel trace lists it)."""
     T = "float" if precision == "mixed" else "double"
     TI = "double" if (stage == 0 and not eng.packed) else T
+    pack = "true" if eng.packed else "false"
     if cls == 0:
-        return f"heom_stage_kernel<{T}, {TI}, {T}, {stage}>"
+        return f"heom_stage_kernel<{T}, {TI}, {T}, {stage}, {pack}>"
     mfma = "true" if (cls == 2 and precision == "mixed") else "false"
     herm = "true" if eng.hermitian else "false"
-    return f"heom_proj_kernel<{T}, {TI}, {T}, {stage}, {mfma}, {herm}>"
+    return f"heom_proj_kernel<{T}, {TI}, {T}, {stage}, {mfma}, {herm}, {pack}>"
 
 
 def measured_traffic(name, workload, packed):
