@@ -320,7 +320,8 @@ def main():
         v64, t64, finals = fp64_plain(insts, dt, args.warmup, args.steps, torch)
         pops, cohs = zip(*[max_errs(a, b) for a, b in zip(mixed_final, finals)])
         out["fp64_plain"] = {"value": v64, "unit": "ADO-updates/s", "seconds": t64,
-                             "kernel": "heom_stage_kernel<double>, one instance per plan, streamed"}
+                             "kernel": "FP64 stage kernels (projector or generic class as the instance allows), "
+                                       "one instance per plan, full-matrix state, no LDS residency"}
         out["ratio_vs_fp64_plain"] = value / v64
         out["error_vs_fp64"] = {"population": max(pops), "coherence": max(cohs),
                                 "steps": args.warmup + args.steps, "instances": len(insts)}
