@@ -122,10 +122,11 @@ int pyqed_heom_tables_build(int32_t K, int32_t L, int64_t* keys, int32_t* plus, 
  * dt, starting at global step index `step0` (time step0*dt).
  * Replaces the time loop of DEOMSolver.run and rk4() (deom.py:725-766,
  * 1072-1114).
- *   y          device, [n_elems] complex double: the hierarchy state, instance
- *              after instance, ADO after ADO (full d x d matrices, or upper
- *              triangles with PYQED_PLAN_PACKED), updated in place (ADO 0 of
- *              each instance is the reduced rho).
+ *   y          device, [n_elems] complex double: the hierarchy state; instance
+ *              i starts at plan_layout's state_offset[i] (instance starts are
+ *              aligned to 128 bytes), then ADO after ADO (full d x d matrices,
+ *              or upper triangles with PYQED_PLAN_PACKED), updated in place
+ *              (ADO 0 of each instance is the reduced rho).
  *   workspace  device, >= ws_bytes for `precision` (plan_query); scratch.
  *   field      device [(step0+nsteps)*6] double or NULL: for step s,
  *              field[6s+0..2] = f_s(t), f_s(t+dt/2), f_s(t+dt) and
@@ -141,8 +142,10 @@ int pyqed_heom_run(pyqed_heom_plan* plan, int32_t precision, void* y, void* work
 
 /* Evaluate the right-hand side once: out = RHS(x) at time index `step`
  * stage column `col` (0: t, 1: t+dt/2, 2: t+dt) in FP64 (device pointers,
- * full matrices of every ADO: sum over instances of nmax*d*d complex double,
- * instance after instance, also for PYQED_PLAN_PACKED plans).  Replaces
+ * full matrices of every ADO, also for PYQED_PLAN_PACKED plans: instance i
+ * starts at element off_i, off_0 = 0, off_{i+1} = off_i + nmax_i*d_i*d_i
+ * rounded up to a multiple of 16 -- the layout of y without
+ * PYQED_PLAN_PACKED).  Replaces
  * rem_cal (deom.py:667-673); used to check the operator against the dense
  * generator. */
 int pyqed_heom_rhs(pyqed_heom_plan* plan, const void* x, void* out, const double* field, int64_t step,

@@ -482,6 +482,12 @@ constexpr int kMaxEnt = 64;  // neighbour entries per ADO handled by the project
 #ifndef PQA_VALU_WAVES
 #define PQA_VALU_WAVES 4  // the same for its FP32 VALU-commutator form (small d, fewer registers)
 #endif
+#ifndef PQA_MFMA_DMIN
+#define PQA_MFMA_DMIN 12  // smallest d whose projector kernel forms the commutator on MFMA
+#endif
+#ifndef PQA_PACK_WAVES
+#define PQA_PACK_WAVES 4  // the same for the packed-state MFMA form
+#endif
 #ifndef PQA_CHUNK
 #define PQA_CHUNK 2
 #endif
@@ -600,8 +606,13 @@ __device__ __forceinline__ void proj_header(const StageParams& p, const InstDev&
   }
 }
 
+// Minimum waves per SIMD each form of the projector kernel is compiled for.
+constexpr int proj_waves(bool mfma, bool f32, bool pack) {
+  return mfma ? (pack ? PQA_PACK_WAVES : PQA_PROJ_WAVES) : (f32 ? PQA_VALU_WAVES : PQA_PROJ_WAVES);
+}
+
 template <typename T, typename TI, typename TO, int STAGE, bool MFMA, bool HERM, bool PACK>
-__global__ __launch_bounds__(kThreads, (MFMA || !std::is_same<T, float>::value) ? PQA_PROJ_WAVES : PQA_VALU_WAVES)
+__global__ __launch_bounds__(kThreads, proj_waves(MFMA, std::is_same<T, float>::value, PACK))
 void heom_proj_kernel(StageParams p) {
   extern __shared__ __align__(16) char smem[];
   const int tid = threadIdx.x, lane = tid & 63;
@@ -614,9 +625,11 @@ void heom_proj_kernel(StageParams p) {
   char* wbase = smem + proj_block_bytes(p.d2pad, p.Kmax, p.Lmax, sizeof(T), PACK);
   WaveLds<T> w;
   {
-    char* b = wbase + wv * proj_wave_bytes(p.d2pad, p.rowbuf, p.colbuf, sizeof(T), PACK);
-    w.Tt = (cpx<T>*)(b + proj_wave_bytes(p.d2pad, p.rowbuf, p.colbuf, sizeof(T), false));
+    // packed mode: the stage-output tile; the MFMA form reuses the own-matrix buffer for it (the
+    // own matrix is dead once the commutator and damping are formed)
+    char* b = wbase + wv * proj_wave_bytes(p.d2pad, p.rowbuf, p.colbuf, sizeof(T), PACK && !MFMA);
     w.X = (cpx<T>*)b;
+    w.Tt = MFMA ? w.X : (cpx<T>*)(b + proj_wave_bytes(p.d2pad, p.rowbuf, p.colbuf, sizeof(T), false));
     w.R = w.X + p.d2pad;
     w.Cc = w.R + p.rowbuf;
     w.cl = w.Cc + p.colbuf;
@@ -657,12 +670,11 @@ void heom_proj_kernel(StageParams p) {
       nent_n = ro_int(p.rowptr, I.rowptr_off + an + 1) - r0;
       en_n = make_int2(0, 0);
       if (lane < nent_n) en_n = p.ent[I.ent_off + r0 + lane];
-      if (MFMA) {
+      // own matrix: its first 256 elements (all of it for d <= 16)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int e = lane + 64 * q;
-          if (e < d2) xt_n[q] = xi[(int64_t)an * d2 + e];
-        }
+      for (int q = 0; q < 4; ++q) {
+        const int e = lane + 64 * q;
+        if (e < d2) xt_n[q] = xi[(int64_t)an * d2 + e];
       }
     };
     if (wi.ado0 + wv < aend) prefetch(wi.ado0 + wv);
@@ -671,15 +683,12 @@ void heom_proj_kernel(StageParams p) {
       // ---- neighbour list and own matrix (prefetched)
       const int nent = nent_n;
       const int2 en = en_n;
-      if (MFMA) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int e = lane + 64 * q;
-          if (e < d2) w.X[e + (int)(((uint32_t)e * dmag) >> 20)] = cvt<T>(mk(xt_n[q].x, xt_n[q].y));
-        }
-      } else {
-        for (int e = lane; e < d2; e += 64) w.X[e + (int)(((uint32_t)e * dmag) >> 20)] = ldc<T>(xi, base + e);
+      for (int q = 0; q < 4; ++q) {
+        const int e = lane + 64 * q;
+        if (e < d2) w.X[e + (int)(((uint32_t)e * dmag) >> 20)] = cvt<T>(mk(xt_n[q].x, xt_n[q].y));
       }
+      for (int e = 256 + lane; e < d2; e += 64) w.X[e + (int)(((uint32_t)e * dmag) >> 20)] = ldc<T>(xi, base + e);
       // ---- neighbour list in site order, with coefficients
       const int k = en.y & 0xffff, nf = (en.y >> 16) & 0x7f;
       const bool plus = en.y < 0;
@@ -736,14 +745,19 @@ void heom_proj_kernel(StageParams p) {
       // ---- commutator and damping (while the neighbour loads are in flight)
       if (MFMA) {
         floatx4 aR = {0.f, 0.f, 0.f, 0.f}, aI = {0.f, 0.f, 0.f, 0.f};
+        // H is read from LDS for every ADO rather than kept in registers across the item's ADOs
+        // (an opaque zero offset stops the compiler from hoisting the loads out of the ADO loop)
+        int hz = 0;
+        asm volatile("" : "+v"(hz));
+        const cpx<T>* sHa = sH + hz;
 #pragma unroll
         for (int kc = 0; kc < 4; ++kc) {
           const int kk = 4 * kc + gk;
           const bool ok = gi < d && kk < d;
           const cpx<T> xb = ok ? w.X[kk * rs + gi] : mk<T>(0, 0);  // B-layout X[kk][gi]
           const cpx<T> xa = ok ? w.X[gi * rs + kk] : mk<T>(0, 0);  // A-layout X[gi][kk]
-          const cpx<T> ha = ok ? sH[gi * rs + kk] : mk<T>(0, 0);   // A-layout H[gi][kk]
-          const cpx<T> hb = ok ? sH[kk * rs + gi] : mk<T>(0, 0);   // B-layout H[kk][gi]
+          const cpx<T> ha = ok ? sHa[gi * rs + kk] : mk<T>(0, 0);  // A-layout H[gi][kk]
+          const cpx<T> hb = ok ? sHa[kk * rs + gi] : mk<T>(0, 0);  // B-layout H[kk][gi]
           const float xbr = (float)xb.x, xbi = (float)xb.y, xar = (float)xa.x, xai = (float)xa.y;
           const float har = (float)ha.x, hai = (float)ha.y, hbr = (float)hb.x, hbi = (float)hb.y;
           // + H X
@@ -817,13 +831,11 @@ void heom_proj_kernel(StageParams p) {
           }
           wave_sync();
           cpx<T> au[3];
-          int ij[3];
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
             const int pk = lane + 64 * q;
             if (pk < np) {
-              ij[q] = sIJ[pk];
-              const int i = ij[q] >> 8, j = ij[q] & 255;
+              const int ij = sIJ[pk], i = ij >> 8, j = ij & 255;
               au[q] = w.Tt[i * rs + j];
               neighbours(au[q], i, j);
             }
@@ -832,9 +844,11 @@ void heom_proj_kernel(StageParams p) {
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
             const int pk = lane + 64 * q;
-            if (pk < np)
-              output(ij[q] >> 8, ij[q] & 255,
+            if (pk < np) {
+              const int ij = sIJ[pk];
+              output(ij >> 8, ij & 255,
                      rk4_upper<STAGE>(p, I.poff + (int64_t)a * I.dp + pk, (double)au[q].x, (double)au[q].y, pr[q]));
+            }
           }
         } else {
           for (int p0 = 0; p0 < np; p0 += 256) {
@@ -1447,8 +1461,9 @@ size_t stream_lds_bytes(const pyqed_heom_plan* P, size_t tsize) {
 
 size_t proj_lds_bytes(const pyqed_heom_plan* P, int c, size_t tsize) {
   const auto& q = P->pd[c];
+  const bool mfma = c == 2 && tsize == sizeof(float);  // the MFMA form keeps its tile in the own-matrix buffer
   return proj_block_bytes(q.d2pad, q.Kmax, q.Lmax, tsize, P->packed) +
-         (kThreads / 64) * proj_wave_bytes(q.d2pad, q.rowbuf, q.colbuf, tsize, P->packed);
+         (kThreads / 64) * proj_wave_bytes(q.d2pad, q.rowbuf, q.colbuf, tsize, P->packed && !mfma);
 }
 
 size_t pipe_lds_bytes(const pyqed_heom_plan* P, int c) {
@@ -1614,6 +1629,11 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     I.nmax = t.nmax;
     I.flags = (s.sdip ? F_SDIP : 0) | (dense ? (s.cmat ? F_CDIP : 0) : (s.cdiag ? F_CDIP : 0)) | (dense ? F_DENSE : 0);
     I.grp = group_size(I.d2);
+    // every instance starts on a 128-byte boundary of the stage buffers (16 FP32 complex elements)
+    // and of the packed FP64 state (8 elements), so that its rows fall on whole cache lines as often
+    // as d allows
+    off = (off + 15) & ~int64_t(15);
+    soff = (soff + 7) & ~int64_t(7);
     I.off = off;
     I.dp = P->packed ? s.d * (s.d + 1) / 2 : I.d2;
     I.poff = P->packed ? soff : off;
@@ -1681,7 +1701,8 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
       }
       // projector path: at most kMaxEnt neighbour entries per ADO (K plus + min(K, L) minus)
       if (proj && s.K + std::min(s.K, s.L) <= kMaxEnt) {
-        I.path = (s.d >= 12 && s.d <= 16) ? 2 : 1;
+        // MFMA commutator (16x16 tiles, d padded to 16) for PQA_MFMA_DMIN <= d <= 16, VALU otherwise
+        I.path = (s.d >= PQA_MFMA_DMIN && s.d <= 16) ? 2 : 1;
         I.site_off = (int)sites.size();
         bool sorted_sites = true;
         for (int k = 0; k < s.K; ++k) {

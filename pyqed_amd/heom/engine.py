@@ -351,7 +351,8 @@ class HeomEngine:
 
     def rhs(self, x, field=None, step=0, col=0):
         """FP64 right-hand side of full hierarchy matrices x (device complex128, every ADO of every
-        instance as a full d x d matrix, instance after instance; also for packed engines)."""
+        instance as a full d x d matrix, in the layout of y without packing: instance starts rounded
+        up to multiples of 16 elements; also for packed engines)."""
         torch = self.torch
         out = torch.empty_like(x)
         _lib.check(self.L.pyqed_heom_rhs(self.plan, x.data_ptr(), out.data_ptr(),
