@@ -13,10 +13,15 @@ Reference: pyqed/oqs.py, pyqed/phys.py, pyqed/superoperator.py (binggu56/pyqed 0
   superoperator       superoperator.py:29-58, 200-251 (kron(a, I) - kron(I, a^T) etc.)
   redfield_tensor     oqs.py:519-570
   run_redfield        oqs.py:364-459 (_redfield): observables (Nt, n_e) in the eigenbasis
+  correlation_*       oqs.py:717-791 (_correlation_2p_1t), 1193-1327 (LindbladSolver.correlation_*):
+                      quantum regression with the same RK4; 3op_2t keeps tau = 0..(Ntau-1)dt (the
+                      reference's assignment of Ntau+1 values into Ntau columns raises)
 
 What pins them (tests/test_oracle_oqs.py): expm of the kron-form Liouvillian, the analytic
 two-level decay and dephasing laws, RK4's fourth-order convergence, trace and Hermiticity
-preservation, detailed balance of the Redfield populations.  The reference's _lindblad cannot be
+preservation, detailed balance of the Redfield populations; for the correlation functions, expm of the
+Liouvillian applied to the regression state and the closed-form two-level correlation
+<sigma+(t) sigma->  = exp((i w - G/2) t).  The reference's _lindblad cannot be
 used for golden vectors: with numpy 2 / scipy 1.15 its csr state makes np.dot(dense, csr) return
 object arrays and the coherent term vanishes; the dense semantics restated here are the intended
 ones.  redfield_tensor has no golden vector either (parity unpinned beyond the mathematical
@@ -64,6 +69,26 @@ def run(H, c_ops, rho0, dt, Nt, e_ops=()):
         rholist.append(rho.copy())
         obs[k + 1] = [obs_dm(rho, e) for e in e_ops]
     return dict(observables=obs, rholist=rholist, times=np.arange(Nt + 1) * dt)
+
+
+def correlation_2op_1t(H, c_ops, rho0, a_op, b_op, dt, Nt):
+    """<A(t)B> = Tr[A rho(t)], rho(0) = B rho0, at t = dt..Nt*dt (oqs.py:717-791)."""
+    return run(H, c_ops, np.asarray(b_op) @ np.asarray(rho0), dt, Nt, [a_op])["observables"][1:, 0]
+
+
+def correlation_3op_1t(H, c_ops, rho0, a_op, b_op, c_op, dt, Nt):
+    """<A B(t) C> = Tr[B rho(t)], rho(0) = C rho0 A, at t = 0..Nt*dt (oqs.py:1224-1242)."""
+    start = np.asarray(c_op) @ np.asarray(rho0) @ np.asarray(a_op)
+    return run(H, c_ops, start, dt, Nt, [b_op])["observables"][:, 0]
+
+
+def correlation_3op_2t(H, c_ops, rho0, a_op, b_op, c_op, dt, Nt, Ntau):
+    """<A(t) B(t+tau) C(t)>, [Nt, Ntau]: row j at t = (j+1)dt, column m at tau = m*dt (oqs.py:1263-1297)."""
+    rhos = run(H, c_ops, rho0, dt, Nt)["rholist"]
+    out = np.zeros((Nt, Ntau), complex)
+    for j, rho in enumerate(rhos):
+        out[j] = correlation_3op_1t(H, c_ops, rho, a_op, b_op, c_op, dt, Ntau)[:Ntau]
+    return out
 
 
 def run_driven(H, rho0, c_ops, dt, Nt, t0=0.0, e_ops=(), legacy=False):

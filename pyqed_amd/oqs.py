@@ -83,6 +83,64 @@ class LindbladSolver:
         _, _, final = propagate(np.array(gens), rho0s, dt, Nt, fields, (), keep_states=False)
         return final
 
+    # ---------------------------------------------------------------- correlation functions
+    # Quantum regression on the GPU Liouville kernel (oqs.py:717-791, 1193-1327).  The propagations
+    # of a two-time correlation (one per t) run as one batched launch.
+
+    def _static_generator(self):
+        if isinstance(self.H, list):
+            raise NotImplementedError("correlation functions need a static Hamiltonian")
+        return lindblad_generator(np.asarray(self.H, np.complex128), self.c_ops or [])[None]
+
+    def correlation_2op_1t(self, rho0, a_op, b_op, dt, Nt, output='cor.dat'):
+        """<A(t)B> = Tr[A e^{Lt}(B rho0)] at t = dt, ..., Nt*dt (oqs.py:1193-1222 -> 717-791): Nt
+        values.  As the reference, lines 't value' are written to `output` (None: no file)."""
+        A = np.asarray(a_op, np.complex128)
+        start = np.asarray(b_op, np.complex128) @ np.asarray(rho0, np.complex128)
+        obs, _, _ = propagate(self._static_generator(), start[None], dt, Nt, None, [A], keep_states=False)
+        cor = obs[0, 1:, 0].copy()
+        if output is not None:
+            with open(output, 'w') as f:
+                t = 0.0
+                for k in range(Nt):
+                    t += dt
+                    f.write('{} {} \n'.format(t, cor[k]))
+        return cor
+
+    def correlation_3op_1t(self, rho0, oplist, dt=0.005, Nt=1):
+        """<A B(t) C> = Tr[B e^{Lt}(C rho0 A)] at t = 0, dt, ..., Nt*dt (oqs.py:1224-1242): Nt+1 values."""
+        a_op, b_op, c_op = (np.asarray(o, np.complex128) for o in oplist)
+        start = c_op @ np.asarray(rho0, np.complex128) @ a_op
+        obs, _, _ = propagate(self._static_generator(), start[None], dt, Nt, None, [b_op], keep_states=False)
+        return obs[0, :, 0]
+
+    def correlation_4op_1t(self, rho0, ops, dt, nt):
+        """<A B(t) C(t) D> (oqs.py:1299-1313, the definition that takes effect): correlation_3op_1t
+        with [A, BC, D]."""
+        if len(ops) != 4:
+            raise ValueError('Number of operators is not 4.')
+        a, b, c, d = ops
+        return self.correlation_3op_1t(rho0, [a, np.asarray(b) @ np.asarray(c), d], dt, nt)
+
+    def correlation_3op_2t(self, rho0, ops, dt, Nt, Ntau):
+        """<A(t) B(t+tau) C(t)> as [Nt, Ntau] (oqs.py:1263-1297): row j is t = (j+1) dt (the states
+        after each step), column m is tau = m dt.  The Nt regression propagations C rho(t) A run in
+        one launch.  (The reference stores the Ntau+1 values of each propagation in a row of width
+        Ntau, which raises; the first Ntau values, tau = 0 .. (Ntau-1) dt, are kept here.)"""
+        a_op, b_op, c_op = (np.asarray(o, np.complex128) for o in ops)
+        gen = self._static_generator()
+        _, states, _ = propagate(gen, np.asarray(rho0, np.complex128)[None], dt, Nt, None, ())
+        starts = c_op @ states[0] @ a_op
+        obs, _, _ = propagate(gen, starts, dt, Ntau, None, [b_op], keep_states=False)
+        return obs[:, :Ntau, 0]
+
+    def correlation_4op_2t(self, rho0, ops, dt, nt, ntau):
+        """<A(t) B(t+tau) C(t+tau) D(t)> (oqs.py:1315-1327): correlation_3op_2t with [A, BC, D]."""
+        if len(ops) != 4:
+            raise ValueError('Number of operators is not 4.')
+        a, b, c, d = ops
+        return self.correlation_3op_2t(rho0, [a, np.asarray(b) @ np.asarray(c), d], dt, nt, ntau)
+
 
 def redfield_tensor(H, a_ops, spectra, secular=False):
     """Bloch-Redfield generator in the eigenbasis of H (oqs.py:519-570); returns (R, evecs) with R a

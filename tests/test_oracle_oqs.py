@@ -144,3 +144,33 @@ def test_product_generators_match_oracle():
     R1, e1 = oqs.redfield_tensor(H, [a], [S])
     R2, e2 = OL.redfield_tensor(H, [a], [S])
     assert np.abs(R1 - R2).max() < 1e-13 and np.abs(e1 - e2).max() < 1e-13
+
+
+def test_correlation_closed_form_two_level():
+    """<sigma+(t) sigma-(0)> for H = diag(0, w), c = sqrt(G)|0><1|, rho0 = |1><1|: exp((i w - G/2) t)."""
+    w, G, dt, Nt = 1.3, 0.4, 0.01, 300
+    H = np.diag([0.0, w]).astype(complex)
+    sm = np.array([[0, 1], [0, 0]], complex)
+    rho0 = np.diag([0.0, 1.0]).astype(complex)
+    g = OL.correlation_2op_1t(H, [np.sqrt(G) * sm], rho0, sm.conj().T, sm, dt, Nt)
+    t = np.arange(1, Nt + 1) * dt
+    assert np.abs(g - np.exp((1j * w - G / 2) * t)).max() < 2e-9  # RK4 global error at dt = 0.01
+    # the three-operator form with A = 1 is the same correlation including t = 0
+    g3 = OL.correlation_3op_1t(H, [np.sqrt(G) * sm], rho0, np.eye(2), sm.conj().T, sm, dt, Nt)
+    assert abs(g3[0] - 1) < 1e-15 and np.abs(g3[1:] - g).max() < 1e-15
+
+
+def test_correlation_3op_2t_against_expm():
+    rng = np.random.default_rng(7)
+    d, dt, Nt, Ntau = 3, 0.02, 6, 9
+    H = _rand_herm(d, rng)
+    c = [0.3 * (rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d)))]
+    a, b, cc = (rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d)) for _ in range(3))
+    rho0 = np.diag([0.5, 0.3, 0.2]).astype(complex)
+    got = OL.correlation_3op_2t(H, c, rho0, a, b, cc, dt, Nt, Ntau)
+    L = OL.superoperator(H, c)
+    for j in range(Nt):
+        rho_t = OL.expm_evolve(L, rho0, (j + 1) * dt)
+        for m in range(Ntau):
+            ref = np.trace(b @ OL.expm_evolve(L, cc @ rho_t @ a, m * dt))
+            assert abs(got[j, m] - ref) < 1e-8 * max(1.0, abs(ref))  # RK4 error at dt = 0.02
