@@ -19,7 +19,11 @@ Printed JSON line (rank 0):
   cpu_baseline numpy FP64 restatement (the oracle), one thread, on a stratified
                sample of the same instances
 
-usage: python bench.py [--steps K] [--warmup W] [--workload c3|c2|c5|c1]
+usage: python bench.py [--steps K] [--warmup W] [--workload c3|c2|c5|c1|legacy]
+
+--workload legacy measures the legacy single-exponential HEOM of oqs.HEOMSolver instead: a
+temperature x reorganization sweep of the spin-boson example, one Euler step of every instance per
+step, roofline against the FP64 peak.
 """
 import argparse
 import json
@@ -193,13 +197,104 @@ def stream_probe(torch):
     return out
 
 
+LEGACY_METRIC = "ADO-updates/s, legacy single-exponential HEOM (oqs.HEOMSolver.run, forward Euler, FP64), 1 GPU"
+FP64_PEAK_TFLOPS = 78.6
+
+
+def legacy_workload(n_inst):
+    """The spin-boson of pyqed's examples/heom.py:77-96 (H = -sx/2 - sz/2, S = sz, e_ops = [sz],
+    rho0 = |1><1|, dt = 0.02, cutoff 5, nado 5), swept over n_inst (temperature, reorganization) points."""
+    sx = np.array([[0, 1], [1, 0]], complex)
+    sz = np.diag([1.0, -1.0]).astype(complex)
+    H = -0.5 * sx - 0.5 * sz
+    rho0 = np.diag([0.0, 1.0]).astype(complex)
+    k = np.arange(n_inst)
+    temperature = 300.0 + 600.0 * (k % 64) / 63.0
+    reorg = 0.05 + 0.35 * (k // 64 % 64) / 63.0
+    return dict(H=H, S=sz, e_ops=[sz], rho0=rho0, dt=0.02, cutoff=5.0, nado=5, temperature=temperature,
+                reorganization=reorg)
+
+
+def legacy_flops(d, nado):
+    """FP64 flops of one step of one instance: ADO 0 takes two commutators (4 complex d x d products),
+    ADOs 1..nado-2 three (6 products: [H, X_n], [S, X_n+1], S X_n-1 and X_n-1 S); 8 d^3 each."""
+    return 8 * d ** 3 * (4 + 6 * (nado - 2))
+
+
+def legacy_cpu_baseline(w, budget_s=12.0):
+    from oracle import legacy_heom as LH
+    n_done, t0, nt = 0, time.perf_counter(), 200
+    while time.perf_counter() - t0 < budget_s:
+        i = (n_done * 997) % len(w["temperature"])
+        gamma, D0 = LH.run_params(w["temperature"][i], w["cutoff"], w["reorganization"][i])
+        LH.heom_run(w["H"], w["rho0"], w["S"], w["e_ops"], gamma, D0, w["nado"], w["dt"], nt)
+        n_done += 1
+    el = time.perf_counter() - t0
+    return {"value": n_done * nt * (w["nado"] - 1) / el, "unit": "ADO-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{n_done} instances x {nt} steps of the same sweep (numpy FP64 restatement, one thread)"}
+
+
+def bench_legacy(args, torch):
+    from pyqed_amd import _lib
+    from pyqed_amd.oqs import legacy_run_params
+    L = _lib.lib()
+    w = legacy_workload(args.n_inst)
+    n, d, nado = args.n_inst, w["H"].shape[0], w["nado"]
+    gamma, D0 = legacy_run_params(w["temperature"], w["cutoff"], w["reorganization"])
+    params = np.stack([np.broadcast_to(gamma, (n,)), D0.real, D0.imag], axis=1)
+    dev = torch.device("cuda", 0)
+    A = torch.as_tensor(np.array(np.broadcast_to(w["H"], (n, d, d)), order="C"), device=dev)
+    S = torch.as_tensor(np.array(np.broadcast_to(w["S"], (n, d, d)), order="C"), device=dev)
+    E = torch.as_tensor(np.array(w["e_ops"]), device=dev)
+    par = torch.as_tensor(params, device=dev)
+    ado0 = np.zeros((n, nado, d, d), complex)
+    ado0[:, 0] = w["rho0"]
+    ado = torch.as_tensor(ado0, device=dev)
+    obs = torch.empty((n, max(args.steps, args.warmup, 1), 1), dtype=torch.complex128, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def launch(nt):
+        _lib.check(L.pyqed_legacy_heom_run(0, d, nado, n, A.data_ptr(), S.data_ptr(), None, par.data_ptr(),
+                                           ado.data_ptr(), nt, w["dt"], 1, E.data_ptr(), obs.data_ptr(),
+                                           stream.cuda_stream))
+
+    launch(args.warmup)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    launch(args.steps)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = ev[0].elapsed_time(ev[1])
+    flops = legacy_flops(d, nado) * n * args.steps
+    achieved = flops / (launch_ms / 1e3) / 1e12
+    out = {
+        "metric": LEGACY_METRIC, "value": n * (nado - 1) * args.steps / wall, "unit": "ADO-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (pyqed examples/heom.py spin-boson over a temperature x reorganization grid)",
+        "config": {"workload": f"legacy spin-boson sweep: {n} instances, d={d}, nado={nado}, dt={w['dt']}, "
+                               f"one observable per step, {args.steps} steps in one launch", "instances": n},
+        "roofline": {"bound": "mfma", "kernel": f"legacy_heom_kernel<false, {'true' if d * d <= 256 else 'false'}>",
+                     "launch_ms": launch_ms,
+                     "flops_per_launch": flops, "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None},
+    }
+    if not args.no_cpu:
+        print("[bench] cpu baseline leg", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = legacy_cpu_baseline(w)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; 2000 for --workload legacy)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c5"])
-    ap.add_argument("--n-inst", type=int, default=4096)
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c5", "legacy"])
+    ap.add_argument("--n-inst", type=int, default=None, help="instances (default 4096; 65536 for legacy)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--state", default="auto", choices=["auto", "full"],
                     help="RK4 state layout: auto = upper triangles for Hermitian hierarchies, full = full matrices")
@@ -214,6 +309,11 @@ def main():
     ap.add_argument("--stream-probe", action="store_true",
                     help="only measure achievable HBM bandwidth: device copies and a 3-read/2-write stream")
     args = ap.parse_args()
+    legacy = args.workload == "legacy"
+    if args.steps is None:
+        args.steps = 2000 if legacy else 20
+    if args.n_inst is None:
+        args.n_inst = 65536 if legacy else 4096
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the propagator has no multi-GPU path "
               "(see DESIGN.md for how the batch would shard).", file=sys.stderr)
@@ -232,6 +332,9 @@ def main():
         return
     if args.stream_probe:
         print(json.dumps(stream_probe(torch)), flush=True)
+        return
+    if legacy:
+        bench_legacy(args, torch)
         return
 
     insts, dt, desc = instances(args.workload, args.n_inst)

@@ -218,6 +218,25 @@ int pyqed_liouville_adjoint(int32_t N, int32_t nterm, const void* A, int32_t n_i
                             const void* rhoT_bar, double* grad_field, void* grad_A, void* rho0_bar,
                             void* ckpt, int64_t ckpt_bytes, void* stream);
 
+/* Legacy single-exponential HEOM of oqs.HEOMSolver, forward Euler with the
+ * reference's in-place sweep over the hierarchy (ADO n sees ADO n-1 of the same
+ * step and ADO n+1 of the previous one).  One workgroup per instance; the
+ * whole hierarchy is held in LDS, so (nado + 4) * M^2 * 16 bytes must fit.
+ *   mode 0 replaces _heom (pyqed/oqs.py:1808-1874, called by HEOMSolver.run,
+ *     oqs.py:1360-1364): M = d, A = H, S = c_ops[0], Sp unused,
+ *     params[i] = (gamma, Re D0, Im D0), ado [n_inst][nado][d][d] starts with
+ *     rho0 in ADO 0; obs_out [n_inst][nt][n_obs] receives Tr(e_ops[o] rho(t))
+ *     after every step (e_ops shared, [n_obs][d][d]).
+ *   mode 1 replaces _heom_propagator (oqs.py:1876-1935, HEOMSolver.propagator,
+ *     oqs.py:1366-1370): M = d^2, A = L0, S = Sm, Sp = the commutator and
+ *     anticommutator superoperators of S, params[i] = (gamma, a, b), ado the
+ *     propagator blocks u [n_inst][nado][M][M] (u[0] = identity); n_obs = 0.
+ * A, S, Sp are device [n_inst][M][M] complex double; params is a device
+ * [n_inst][3] double array; ado is advanced in place by nt steps of dt. */
+int pyqed_legacy_heom_run(int32_t mode, int32_t M, int32_t nado, int32_t n_inst, const void* A, const void* S,
+                          const void* Sp, const double* params, void* ado, int64_t nt, double dt, int32_t n_obs,
+                          const void* e_ops, void* obs_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

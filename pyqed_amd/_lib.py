@@ -26,7 +26,7 @@ SYMBOLS = [
     "pyqed_heom_tables_export", "pyqed_heom_tables_size", "pyqed_heom_tables_build", "pyqed_heom_run", "pyqed_heom_rhs", "pyqed_heom_plan_stats",
     "pyqed_heom_plan_classes", "pyqed_heom_plan_set_tiling", "pyqed_heom_plan_get_tiling", "pyqed_heom_plan_set_variant",
     "pyqed_heom_plan_class_state", "pyqed_heom_plan_set_timing", "pyqed_heom_plan_read_timing",
-    "pyqed_liouville_run", "pyqed_liouville_adjoint",
+    "pyqed_liouville_run", "pyqed_liouville_adjoint", "pyqed_legacy_heom_run",
 ]
 
 
@@ -82,6 +82,7 @@ def lib():
     L.pyqed_heom_plan_read_timing.argtypes = [vp, vp, vp]
     L.pyqed_liouville_run.argtypes = [i32, i32, vp, i32, vp, vp, i64, dbl, i32, vp, vp, vp, i64, vp]
     L.pyqed_liouville_adjoint.argtypes = [i32, i32, vp, i32, vp, vp, i64, dbl, i32, vp, vp, vp, vp, vp, i64, vp]
+    L.pyqed_legacy_heom_run.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, vp, i64, dbl, i32, vp, vp, vp]
     for name in SYMBOLS:
         if name != "pyqed_last_error":
             getattr(L, name).restype = C.c_int

@@ -1,0 +1,221 @@
+// Single-exponential (high-temperature) HEOM of pyqed's legacy solver oqs.HEOMSolver
+// (pyqed/oqs.py:1332-1372, 1808-1953): forward Euler with the reference's in-place sweep.
+//
+// run (mode 0): ADOs X_n, d x d, n = 0..nado-1.  Per step (oqs.py:1849-1862):
+//   X_0 <- X_0 - i [H, X_0] dt - [S, X_1] dt
+//   for n = 1..nado-2:
+//     X_n <- X_n + (-i [H, X_n] - [S, X_{n+1}] - n gamma X_n
+//                   + n (Re D0 [S, X_{n-1}] + i Im D0 {S, X_{n-1}})) dt
+//   with X_{n-1} already updated in this step and X_{n+1} not yet; X_{nado-1} is never updated
+//   (the truncation, zero when started from zero).  Observables Tr(E_o X_0) after every step.
+// propagator (mode 1): blocks U_n, N x N (N = d^2), left products with the superoperators
+//   L0 = H x I - I x H^T, Sm (commutator) and Sp (anticommutator) of S.  Per step
+//   (oqs.py:1920-1932):
+//   U_0 <- U_0 - i L0 U_0 dt - Sm U_1 dt
+//   for n = 0..nado-2:  (n = 0 updates U_0 a second time, as the reference's loop does)
+//     U_n <- U_n + (-i L0 U_n - Sm U_{n+1} - n gamma U_n + n (a Sm + i b Sp) U_{n-1}) dt
+// Both are one update formula, u(n), with the products two-sided (mode 0) or left-only (mode 1).
+// One workgroup owns one instance and keeps every ADO in LDS; the sweep over n is sequential and
+// the elements of one ADO are updated in parallel.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "common.h"
+
+using namespace pqa;
+
+namespace {
+
+constexpr int kMaxThreads = 256;
+
+struct LgParams {
+  int M, nado, n_obs, n_inst;
+  int gsz;            // lanes per instance group: a power of two; gsz < blockDim.x packs several
+                      // instances (blockDim.x / gsz) into one workgroup
+  const double2* A;   // run: H; propagator: L0              [n_inst][M][M]
+  const double2* S;   // run: S; propagator: Sm
+  const double2* P;   // propagator: Sp
+  const double* par;  // [n_inst][3]: gamma, cr, ci (run: Re D0, Im D0; propagator: a, b)
+  double2* ado;       // [n_inst][nado][M][M], advanced in place
+  int64_t nt;
+  double dt;
+  const double2* eops;  // [n_obs][M][M] (run)
+  double2* obs_out;     // [n_inst][nt][n_obs]
+};
+
+// PROP = false: two-sided products (commutators of d x d ADOs); true: left products of N x N blocks.
+// ONE = true: every lane owns at most one element of its instance (M^2 <= gsz), the new value waits
+// in a register; otherwise lanes stride over the elements and new values go through LDS.
+// Instances are packed gsz lanes apart, so a group never straddles a wavefront when gsz <= 64.
+template <bool PROP, bool ONE>
+__global__ __launch_bounds__(kMaxThreads) void legacy_heom_kernel(LgParams p) {
+  extern __shared__ __align__(16) char smem[];
+  const int tid = threadIdx.x, nth = blockDim.x, M = p.M, M2 = M * M, nado = p.nado, gsz = p.gsz;
+  const int g = tid / gsz, li = tid - g * gsz;
+  const int b = blockIdx.x * (nth / gsz) + g;
+  const bool live = b < p.n_inst;  // lanes of a missing instance still take every barrier
+  // per-group LDS: X [nado][M2], Tn [M2] (!ONE), sA, sS [M2], sP [M2] (PROP)
+  const int per = (nado + (ONE ? 0 : 1) + 2 + (PROP ? 1 : 0)) * M2;
+  cpx<double>* X = (cpx<double>*)smem + (size_t)g * per;
+  cpx<double>* Tn = X + (size_t)nado * M2;
+  cpx<double>* sA = Tn + (ONE ? 0 : M2);
+  cpx<double>* sS = sA + M2;
+  cpx<double>* sP = sS + M2;
+  double* red = (double*)((cpx<double>*)smem + (size_t)(nth / gsz) * per);  // [2 * nth]
+  double gamma = 0.0, cr = 0.0, ci = 0.0;
+  double2* gado = p.ado + (size_t)(live ? b : 0) * nado * M2;
+  if (live) {
+    const double2* Ag = p.A + (size_t)b * M2;
+    const double2* Sg = p.S + (size_t)b * M2;
+    for (int e = li; e < M2; e += gsz) {
+      sA[e] = mk(Ag[e].x, Ag[e].y);
+      sS[e] = mk(Sg[e].x, Sg[e].y);
+      if (PROP) sP[e] = mk(p.P[(size_t)b * M2 + e].x, p.P[(size_t)b * M2 + e].y);
+    }
+    for (int e = li; e < nado * M2; e += gsz) X[e] = mk(gado[e].x, gado[e].y);
+    gamma = p.par[3 * b];
+    cr = p.par[3 * b + 1];
+    ci = p.par[3 * b + 2];
+  }
+  const double dt = p.dt;
+  __syncthreads();
+
+  // one update of ADO n from the current LDS contents (n-1 already new, n+1 still old)
+  auto update = [&](int n) {
+    cpx<double>* Xn = X + (size_t)n * M2;
+    const cpx<double>* Y = X + (size_t)(n + 1) * M2;
+    const cpx<double>* Z = X + (size_t)(n > 0 ? n - 1 : 0) * M2;
+    cpx<double> nv = mk(0.0, 0.0);
+    for (int e = li; live && e < M2; e += gsz) {
+      const int i = e / M, j = e - (e / M) * M;
+      cpx<double> h = mk(0.0, 0.0), s = mk(0.0, 0.0), zc = mk(0.0, 0.0), za = mk(0.0, 0.0);
+      for (int k = 0; k < M; ++k) {
+        cmac(h, sA[i * M + k], Xn[k * M + j]);
+        cmac(s, sS[i * M + k], Y[k * M + j]);
+        if (!PROP) {
+          cmsub(h, Xn[i * M + k], sA[k * M + j]);
+          cmsub(s, Y[i * M + k], sS[k * M + j]);
+        }
+        if (n > 0) {
+          if (!PROP) {
+            const cpx<double> l = sS[i * M + k] * Z[k * M + j], r = Z[i * M + k] * sS[k * M + j];
+            zc = zc + (l - r);
+            za = za + (l + r);
+          } else {
+            cmac(zc, sS[i * M + k], Z[k * M + j]);
+            cmac(za, sP[i * M + k], Z[k * M + j]);
+          }
+        }
+      }
+      // -i h - s - n gamma X_n + n (cr zc + i ci za)
+      cpx<double> rhs = mul_mi(h) - s - Xn[e] * ((double)n * gamma);
+      if (n > 0) rhs = rhs + (zc * cr + mk(-ci * za.y, ci * za.x)) * (double)n;
+      nv = Xn[e] + rhs * dt;
+      if (!ONE) Tn[e] = nv;
+    }
+    __syncthreads();
+    if (ONE) {
+      if (live && li < M2) Xn[li] = nv;
+    } else {
+      for (int e = li; live && e < M2; e += gsz) Xn[e] = Tn[e];
+    }
+    __syncthreads();
+  };
+
+  for (int64_t step = 0; step < p.nt; ++step) {
+    update(0);
+    for (int n = PROP ? 0 : 1; n <= nado - 2; ++n) update(n);
+    for (int o = 0; o < p.n_obs; ++o) {
+      // Tr(E_o X_0) = sum_ij E_o[j][i] X_0[i][j], summed over the group's lanes
+      const double2* E = p.eops + (size_t)o * M2;
+      double re = 0.0, im = 0.0;
+      for (int e = li; live && e < M2; e += gsz) {
+        const int i = e / M, j = e - (e / M) * M;
+        const cpx<double> v = mk(E[j * M + i].x, E[j * M + i].y) * X[e];
+        re += v.x;
+        im += v.y;
+      }
+      red[tid] = re;
+      red[nth + tid] = im;
+      __syncthreads();
+      for (int w = gsz / 2; w > 0; w >>= 1) {
+        if (li < w) {
+          red[tid] += red[tid + w];
+          red[nth + tid] += red[nth + tid + w];
+        }
+        __syncthreads();
+      }
+      if (live && li == 0) p.obs_out[((size_t)b * p.nt + step) * p.n_obs + o] = make_double2(red[tid], red[nth + tid]);
+      __syncthreads();
+    }
+  }
+  if (live)
+    for (int e = li; e < nado * M2; e += gsz) gado[e] = make_double2(X[e].x, X[e].y);
+}
+
+template <bool PROP, bool ONE>
+int launch_legacy(const LgParams& p, int blocks, int nth, size_t lds, hipStream_t stream) {
+  if (lds > 65536)
+    PQA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&legacy_heom_kernel<PROP, ONE>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL((legacy_heom_kernel<PROP, ONE>), dim3(blocks), dim3(nth), lds, stream, p);
+  PQA_HIP(hipGetLastError());
+  return PYQED_OK;
+}
+
+}  // namespace
+
+extern "C" int pyqed_legacy_heom_run(int32_t mode, int32_t M, int32_t nado, int32_t n_inst, const void* A,
+                                     const void* S, const void* Sp, const double* params, void* ado, int64_t nt,
+                                     double dt, int32_t n_obs, const void* e_ops, void* obs_out, void* stream) {
+  if (mode != 0 && mode != 1) return fail(PYQED_EINVAL, "legacy_heom_run: mode must be 0 (run) or 1 (propagator)");
+  if (M < 1 || M > 4096 || nado < 2 || n_inst < 1 || !A || !S || !params || !ado || (mode == 1 && !Sp))
+    return fail(PYQED_EINVAL, "legacy_heom_run: need 1 <= M <= 4096, nado >= 2, n_inst >= 1 and every operator");
+  if (nt < 0) return fail(PYQED_EINVAL, "legacy_heom_run: negative step count");
+  if (n_obs < 0 || (n_obs > 0 && (mode != 0 || !e_ops || !obs_out)))
+    return fail(PYQED_EINVAL, "legacy_heom_run: observables need mode 0, e_ops and obs_out");
+  const int M2 = M * M;
+  // M^2 <= 64: one wavefront holds 64 / gsz instances, gsz = M^2 rounded up to a power of two;
+  // larger blocks: one instance per workgroup of 128 or 256 lanes
+  int gsz = 1;
+  while (gsz < M2) gsz *= 2;
+  const int nth = gsz <= 64 ? 64 : (gsz <= 128 ? 128 : 256);
+  if (gsz > nth) gsz = nth;
+  const int groups = nth / gsz;
+  const bool one = M2 <= gsz;
+  const size_t per = (size_t)(nado + (one ? 0 : 1) + 2 + (mode == 1 ? 1 : 0)) * M2;
+  const size_t lds = sizeof(double2) * per * groups + 2 * sizeof(double) * nth;
+  int dev = 0, maxlds = 65536;
+  PQA_HIP(hipGetDevice(&dev));
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0) maxlds = v;
+  }
+  if (lds > (size_t)maxlds)
+    return fail(PYQED_EINVAL, "legacy_heom_run: the hierarchy (nado + 2 to 4 blocks of M^2 complex doubles per "
+                              "instance) exceeds one workgroup's LDS (" + std::to_string(lds) + " > " +
+                              std::to_string(maxlds) + " bytes)");
+  LgParams p;
+  std::memset(&p, 0, sizeof(p));
+  p.M = M;
+  p.nado = nado;
+  p.n_obs = n_obs;
+  p.n_inst = n_inst;
+  p.gsz = gsz;
+  p.A = (const double2*)A;
+  p.S = (const double2*)S;
+  p.P = (const double2*)Sp;
+  p.par = params;
+  p.ado = (double2*)ado;
+  p.nt = nt;
+  p.dt = dt;
+  p.eops = (const double2*)e_ops;
+  p.obs_out = (double2*)obs_out;
+  const int blocks = (n_inst + groups - 1) / groups;
+  hipStream_t st = (hipStream_t)stream;
+  if (mode == 0)
+    return one ? launch_legacy<false, true>(p, blocks, nth, lds, st) : launch_legacy<false, false>(p, blocks, nth, lds, st);
+  return one ? launch_legacy<true, true>(p, blocks, nth, lds, st) : launch_legacy<true, false>(p, blocks, nth, lds, st);
+}

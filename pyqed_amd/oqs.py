@@ -1,8 +1,9 @@
-"""Drop-in Lindblad and Redfield solvers (pyqed/oqs.py) on the MI355X Liouville-space kernels.
+"""Drop-in Lindblad, Redfield and legacy HEOM solvers (pyqed/oqs.py) on the MI355X kernels.
 
 LindbladSolver(H, c_ops, e_ops).run(rho0, dt, Nt, t0=0., e_ops=None, return_result=True) and
 RedfieldSolver(H, c_ops, spectra, e_ops).evolve(rho0, dt, Nt, ...) return the reference's Result
 (observables, rholist, times).  The RK4 is that of pyqed/phys.py:1051-1063 and runs on the GPU.
+HEOMSolver(H, c_ops, e_ops).run / propagator are the single-exponential HEOM of oqs.py:1332-1372.
 
 Documented differences from the reference:
   * the state is dense: the reference's _lindblad converts rho to csr and, with numpy 2, the
@@ -15,6 +16,7 @@ Documented differences from the reference:
 """
 import numpy as np
 
+from . import _lib
 from .liouville import dag, lindblad_generator, op2sop, propagate
 from .mol import Result
 
@@ -201,3 +203,143 @@ class RedfieldSolver:
         result.observables = obs[0, 1:]
         result.rholist = [ev @ states[0, k] @ dag(ev) for k in range(states.shape[1])]
         return result
+
+
+# ------------------------------------------------------------------ legacy single-exponential HEOM
+
+AU2K = 315775.13   # K per hartree (pyqed/units.py:6)
+
+
+def _dense(a):
+    return np.asarray(a.toarray() if hasattr(a, "toarray") else a, np.complex128)
+
+
+def legacy_run_params(temperature, cutoff, reorganization):
+    """(gamma, D0) of _heom (oqs.py:1826-1845): gamma = cutoff, D0 = reorg gamma (coth(gamma/2T) - i),
+    T in energy units; arrays broadcast."""
+    T, gamma, lam = (np.asarray(v, np.float64) for v in (temperature, cutoff, reorganization))
+    return gamma, lam * gamma * (1.0 / np.tanh(gamma / (2.0 * T)) - 1j)
+
+
+def host_c_order(x, dtype=np.complex128):
+    """A C-ordered copy of x.  The kernels index raw row-major buffers, and np.array of a broadcast
+    or transposed view keeps the view's axis order (order='K'), which torch.as_tensor then keeps."""
+    return np.array(x, dtype, order="C")
+
+
+def legacy_heom_launch(mode, A, S, Sp, params, ados, nt, dt, e_ops=()):
+    """Call pyqed_legacy_heom_run.  A, S, Sp: [n_inst, M, M]; params: [n_inst, 3]; ados:
+    [n_inst, nado, M, M] (initial hierarchy).  Returns (final ados, observables [n_inst, nt, n_e])."""
+    torch = _lib.require_gpu()
+    L = _lib.lib()
+    dev = torch.device("cuda", torch.cuda.current_device())
+
+    def dev_c(x):
+        return torch.as_tensor(host_c_order(x), device=dev)
+
+    A_t, S_t = dev_c(A), dev_c(S)
+    P_t = dev_c(Sp) if Sp is not None else None
+    par = torch.as_tensor(host_c_order(params, np.float64), device=dev)
+    ado = dev_c(ados).clone()
+    n_inst, nado, M = ado.shape[0], ado.shape[1], ado.shape[2]
+    if not all(t.is_contiguous() for t in (A_t, S_t, par, ado) + ((P_t,) if P_t is not None else ())):
+        raise ValueError("legacy_heom_launch: device buffers must be contiguous")
+    n_obs = len(e_ops)
+    E = obs = None
+    if n_obs:
+        E = dev_c(np.array([_dense(e) for e in e_ops]))
+        obs = torch.empty((n_inst, max(nt, 1), n_obs), dtype=torch.complex128, device=dev)
+    _lib.check(L.pyqed_legacy_heom_run(mode, M, nado, n_inst, A_t.data_ptr(), S_t.data_ptr(), _lib.ptr(P_t),
+                                       par.data_ptr(), ado.data_ptr(), int(nt), float(dt), n_obs, _lib.ptr(E),
+                                       _lib.ptr(obs), _lib.stream_ptr(torch)))
+    torch.cuda.synchronize()
+    o = obs.cpu().numpy()[:, :nt] if n_obs else np.zeros((n_inst, nt, 0), complex)
+    return ado.cpu().numpy(), o
+
+
+class HEOMSolver:
+    """Single-exponential high-temperature HEOM of pyqed's oqs.HEOMSolver (oqs.py:1332-1372) on the GPU.
+
+    run(rho0, dt, nt, temperature, cutoff, reorganization, nado) -> observables (n_e, nt), the
+    forward-Euler hierarchy of _heom (oqs.py:1808-1874) with temperature in energy units;
+    propagator(dt, nt, temperature, cutoff, reorganization, nado) -> u (nado, d^2, d^2), the
+    superoperator recursion of _heom_propagator (oqs.py:1876-1935) with temperature in kelvin.
+    Both keep the reference's in-place sweep, truncation and printed diagnostics.  Documented
+    differences: e_ops=None means no observables (the reference raises on len(None)); the whole
+    hierarchy of one instance must fit one workgroup's LDS ((nado + 2) d^2 complex doubles for run,
+    (nado + 3) d^4 for propagator, one block more when the block exceeds 256 elements), and larger
+    cases raise."""
+
+    def __init__(self, H=None, c_ops=None, e_ops=None):
+        self.c_ops = c_ops
+        self.e_ops = e_ops
+        self.H = H
+
+    def set_c_ops(self, c_ops):
+        self.c_ops = c_ops
+
+    def set_e_ops(self, e_ops):
+        self.e_ops = e_ops
+
+    def setH(self, H):
+        self.H = H
+
+    def configure(self, c_ops, e_ops):
+        self.c_ops = c_ops
+        self.e_ops = e_ops
+
+    @staticmethod
+    def _announce(T, gamma, reorg, amp, check_label):
+        print('Temperature of the environment = {}'.format(T))
+        print('{} = {}'.format(check_label, gamma / T))
+        if gamma / T > 0.8:
+            print('WARNING: High-Temperature Approximation may fail.')
+        print('Reorganization energy = {}'.format(reorg))
+        print('Amplitude of the fluctuations = {}'.format(amp))
+
+    def _ops(self):
+        return _dense(self.H), _dense(self.c_ops[0])
+
+    def run(self, rho0, dt, nt, temperature, cutoff, reorganization, nado, *, verbose=True):
+        H, S = self._ops()
+        gamma, D0 = legacy_run_params(temperature, cutoff, reorganization)
+        gamma, D0, T = float(gamma), complex(D0), float(temperature)
+        if verbose:
+            self._announce(T, gamma, reorganization, D0, 'Cutoff gamma/(kT)')
+        d = H.shape[0]
+        ados = np.zeros((1, nado, d, d), np.complex128)
+        ados[0, 0] = _dense(rho0)
+        e_ops = list(self.e_ops) if self.e_ops is not None else []
+        _, obs = legacy_heom_launch(0, H[None], S[None], None, [[gamma, D0.real, D0.imag]], ados, nt, dt, e_ops)
+        return obs[0].T.copy()
+
+    def run_batch(self, rho0s, dt, nt, temperature, cutoff, reorganization, nado, return_ados=False):
+        """Many hierarchies in one launch: rho0s [n, d, d]; temperature, cutoff and reorganization are
+        scalars or length-n arrays.  Returns observables [n, n_e, nt] (and the final ADOs
+        [n, nado, d, d] with return_ados)."""
+        H, S = self._ops()
+        rho0s = np.asarray(rho0s, np.complex128)
+        n, d = rho0s.shape[0], H.shape[0]
+        gamma, D0 = (np.broadcast_to(v, (n,)) for v in legacy_run_params(temperature, cutoff, reorganization))
+        params = np.stack([gamma, D0.real, D0.imag], axis=1)
+        ados = np.zeros((n, nado, d, d), np.complex128)
+        ados[:, 0] = rho0s
+        e_ops = list(self.e_ops) if self.e_ops is not None else []
+        fin, obs = legacy_heom_launch(0, np.broadcast_to(H, (n, d, d)), np.broadcast_to(S, (n, d, d)), None, params,
+                                      ados, nt, dt, e_ops)
+        obs = np.transpose(obs, (0, 2, 1)).copy()
+        return (obs, fin) if return_ados else obs
+
+    def propagator(self, dt, nt, temperature, cutoff, reorganization, nado, *, verbose=True):
+        H, S = self._ops()
+        gamma = float(cutoff)
+        T = temperature / AU2K
+        a, b = np.pi * reorganization * T, 0.0
+        if verbose:
+            self._announce(T, gamma, reorganization, a, 'High-Temperature check gamma/(kT)')
+        N = H.shape[0] ** 2
+        u = np.zeros((1, nado, N, N), np.complex128)
+        u[0, 0] = np.eye(N)
+        fin, _ = legacy_heom_launch(1, op2sop(H)[None], op2sop(S)[None], op2sop(S, "anticommutator")[None],
+                                    [[gamma, a, b]], u, nt, dt)
+        return fin[0]

@@ -38,6 +38,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 
 def load_reference():
+    sys.dont_write_bytecode = True  # nothing is written next to the reference sources
     stub = types.ModuleType("numba")
 
     def njit(*args, **kwargs):
