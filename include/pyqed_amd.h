@@ -237,6 +237,23 @@ int pyqed_legacy_heom_run(int32_t mode, int32_t M, int32_t nado, int32_t n_inst,
                           const void* Sp, const double* params, void* ado, int64_t nt, double dt, int32_t n_obs,
                           const void* e_ops, void* obs_out, void* stream);
 
+/* Spectral sums of a diagonalised generator L = U1 diag(lambda) U2:
+ *   out[t][a][b] = sum_j A[a][j] f(lambda_j, s[t]) B[j][b]
+ *   kind 0: f = exp(lambda s)  (time domain); kind 1: f = -1/(lambda + i s) (frequency domain).
+ * Replaces the expansion loops of the eigen-decomposition solvers:
+ * superoperator.Lindblad_solver.evolve / correlation_2op_1t / 2op_1w / 3op_1t / 3op_1w /
+ * 3op_2t (pyqed/superoperator.py:535-739) and RedfieldSolver.propagator's
+ * 'SOS' branch, contract('aj, jk, jb -> abk', U1, E, U2) (pyqed/oqs.py:197-207).
+ *   A   device [nrow][k] complex double      lam device [k] complex double
+ *   B   device [k][ncol] complex double      s   device [ns] double
+ *   out device complex double, element (t, a, b) at t*so_t + a*so_a + b*so_b
+ *   work device workspace of pyqed_spectral_work_bytes(k, ns) bytes (the weights)
+ * nrow <= 65535.  The eigen-decomposition itself runs on the host (LAPACK). */
+int64_t pyqed_spectral_work_bytes(int32_t k, int32_t ns);
+int pyqed_spectral_sum(int32_t kind, int32_t nrow, int32_t k, int32_t ncol, int32_t ns, const void* A,
+                       const void* lam, const void* B, const double* s, void* out, int64_t so_t, int64_t so_a,
+                       int64_t so_b, void* work, int64_t work_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,7 @@ SYMBOLS = [
     "pyqed_heom_plan_classes", "pyqed_heom_plan_set_tiling", "pyqed_heom_plan_get_tiling", "pyqed_heom_plan_set_variant",
     "pyqed_heom_plan_class_state", "pyqed_heom_plan_set_timing", "pyqed_heom_plan_read_timing",
     "pyqed_liouville_run", "pyqed_liouville_adjoint", "pyqed_legacy_heom_run",
+    "pyqed_spectral_work_bytes", "pyqed_spectral_sum",
 ]
 
 
@@ -83,9 +84,12 @@ def lib():
     L.pyqed_liouville_run.argtypes = [i32, i32, vp, i32, vp, vp, i64, dbl, i32, vp, vp, vp, i64, vp]
     L.pyqed_liouville_adjoint.argtypes = [i32, i32, vp, i32, vp, vp, i64, dbl, i32, vp, vp, vp, vp, vp, i64, vp]
     L.pyqed_legacy_heom_run.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, vp, i64, dbl, i32, vp, vp, vp]
+    L.pyqed_spectral_work_bytes.argtypes = [i32, i32]
+    L.pyqed_spectral_sum.argtypes = [i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, i64, i64, i64, vp, i64, vp]
     for name in SYMBOLS:
         if name != "pyqed_last_error":
             getattr(L, name).restype = C.c_int
+    L.pyqed_spectral_work_bytes.restype = C.c_int64
     _lib = L
     return L
 

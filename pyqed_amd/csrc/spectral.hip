@@ -1,0 +1,165 @@
+// Spectral sums of a diagonalised generator, for the eigen-decomposition propagators and
+// correlation functions of pyqed (superoperator.Lindblad_solver, superoperator.py:455-772;
+// RedfieldSolver.propagator, oqs.py:160-215):
+//
+//   out[t][a][b] = sum_j A[a][j] f(lambda_j, s_t) B[j][b]
+//   f = exp(lambda s)          (kind 0, time domain: e^{Lt} = U1 diag(e^{lambda t}) U2)
+//   f = -1 / (lambda + i s)    (kind 1, frequency domain: the Laplace transform of e^{lambda t})
+//
+// Two launches: the weights W[t][j] = f(lambda_j, s_t), then a batched complex GEMM per row a,
+// out_a = W (ns x k) . C_a (k x ncol) with C_a[j][b] = A[a][j] B[j][b] formed as the tile is staged.
+// The output is written through element strides (so_t, so_a, so_b), which gives both the
+// [t][a][b] and the reference's [a][b][t] layouts.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "common.h"
+
+using namespace pqa;
+
+namespace {
+
+constexpr int kTT = 64;   // t rows per tile
+constexpr int kTB = 16;   // b columns per tile
+constexpr int kTJ = 16;   // j per staged chunk
+constexpr int kRT = 4;    // t rows per thread (register block)
+constexpr int kThreads = (kTT / kRT) * kTB;  // 256
+
+__global__ __launch_bounds__(256) void spectral_weights_kernel(int kind, int ns, int k, const double2* lam,
+                                                               const double* s, double2* W) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)ns * k) return;
+  const int t = (int)(idx / k), j = (int)(idx - (int64_t)t * k);
+  const double lr = lam[j].x, li = lam[j].y, st = s[t];
+  double2 w;
+  if (kind == 0) {
+    const double m = exp(lr * st);
+    double sn, cs;
+    sincos(li * st, &sn, &cs);
+    w = make_double2(m * cs, m * sn);
+  } else {
+    // -1 / (lr + i (li + s))
+    const double re = lr, im = li + st, den = re * re + im * im;
+    w = make_double2(-re / den, im / den);
+  }
+  W[idx] = w;
+}
+
+// RB columns per thread: RB = 1 gives a 64 x 16 tile (narrow outputs, e.g. the correlation
+// functions with ncol = 1); RB = 4 a 64 x 64 tile with a 4 x 4 register block per thread, which
+// halves the LDS reads per complex FMA for wide outputs (propagators).
+// TFAST: consecutive lanes take consecutive t (rows tx + 16 r, columns ty + 16 q), so stores are
+// coalesced when t is the contiguous output axis; otherwise consecutive lanes take consecutive b
+// (rows 4 ty + r, columns tx + 16 q).
+template <int RB, bool TFAST>
+__global__ __launch_bounds__(kThreads) void spectral_contract_kernel(int ns, int k, int ncol, const double2* W,
+                                                                      const double2* A, const double2* B,
+                                                                      double2* out, int64_t so_t, int64_t so_a,
+                                                                      int64_t so_b) {
+  constexpr int TBW = kTB * RB;              // tile width in b
+  __shared__ cpx<double> sW[kTJ][kTT + 1];   // W[t0 + r][j0 + c] stored as [c][r]
+  __shared__ cpx<double> sC[kTJ][TBW + 1];   // A[a][j0 + c] * B[j0 + c][b0 + q]
+  const int tid = threadIdx.x, tx = tid % kTB, ty = tid / kTB;
+  const int t0 = blockIdx.x * kTT, b0 = blockIdx.y * TBW, a = blockIdx.z;
+  cpx<double> acc[kRT][RB];
+#pragma unroll
+  for (int r = 0; r < kRT; ++r)
+#pragma unroll
+    for (int q = 0; q < RB; ++q) acc[r][q] = mk(0.0, 0.0);
+  const double2* Arow = A + (int64_t)a * k;
+  for (int j0 = 0; j0 < k; j0 += kTJ) {
+    // stage W: 64 x 16 values, 4 per thread
+    for (int e = tid; e < kTT * kTJ; e += kThreads) {
+      const int r = e / kTJ, c = e - (e / kTJ) * kTJ;
+      const int t = t0 + r, j = j0 + c;
+      double2 v = make_double2(0.0, 0.0);
+      if (t < ns && j < k) v = W[(int64_t)t * k + j];
+      sW[c][r] = mk(v.x, v.y);
+    }
+    // stage C: 16 x TBW values, RB per thread
+    for (int e = tid; e < kTJ * TBW; e += kThreads) {
+      const int c = e / TBW, q = e - (e / TBW) * TBW;
+      const int j = j0 + c, b = b0 + q;
+      cpx<double> v = mk(0.0, 0.0);
+      if (j < k && b < ncol) {
+        const double2 x = Arow[j], y = B[(int64_t)j * ncol + b];
+        v = mk(x.x, x.y) * mk(y.x, y.y);
+      }
+      sC[c][q] = v;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int c = 0; c < kTJ; ++c) {
+      cpx<double> cv[RB], wv[kRT];
+#pragma unroll
+      for (int q = 0; q < RB; ++q) cv[q] = sC[c][(TFAST ? ty : tx) + kTB * q];
+#pragma unroll
+      for (int r = 0; r < kRT; ++r) wv[r] = sW[c][TFAST ? tx + kTB * r : ty * kRT + r];
+#pragma unroll
+      for (int r = 0; r < kRT; ++r)
+#pragma unroll
+        for (int q = 0; q < RB; ++q) cmac(acc[r][q], wv[r], cv[q]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < RB; ++q) {
+    const int b = b0 + (TFAST ? ty : tx) + kTB * q;
+    if (b >= ncol) continue;
+#pragma unroll
+    for (int r = 0; r < kRT; ++r) {
+      const int t = t0 + (TFAST ? tx + kTB * r : ty * kRT + r);
+      if (t < ns)
+        out[(int64_t)t * so_t + (int64_t)a * so_a + (int64_t)b * so_b] = make_double2(acc[r][q].x, acc[r][q].y);
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int64_t pyqed_spectral_work_bytes(int32_t k, int32_t ns) {
+  if (k < 1 || ns < 1) return 0;
+  return (int64_t)k * ns * (int64_t)sizeof(double2);
+}
+
+extern "C" int pyqed_spectral_sum(int32_t kind, int32_t nrow, int32_t k, int32_t ncol, int32_t ns, const void* A,
+                                  const void* lam, const void* B, const double* s, void* out, int64_t so_t,
+                                  int64_t so_a, int64_t so_b, void* work, int64_t work_bytes, void* stream) {
+  if (kind != 0 && kind != 1) return fail(PYQED_EINVAL, "spectral_sum: kind must be 0 (exp) or 1 (resolvent)");
+  if (nrow < 1 || k < 1 || ncol < 1 || ns < 1)
+    return fail(PYQED_EINVAL, "spectral_sum: nrow, k, ncol and ns must be positive");
+  if (nrow > 65535) return fail(PYQED_EINVAL, "spectral_sum: at most 65535 rows per call");
+  if (!A || !lam || !B || !s || !out || !work) return fail(PYQED_EINVAL, "spectral_sum: null pointer");
+  if (work_bytes < pyqed_spectral_work_bytes(k, ns))
+    return fail(PYQED_EINVAL, "spectral_sum: workspace of " + std::to_string(work_bytes) + " bytes < " +
+                                  std::to_string(pyqed_spectral_work_bytes(k, ns)) + " needed");
+  hipStream_t st = (hipStream_t)stream;
+  double2* W = (double2*)work;
+  const int64_t nw = (int64_t)ns * k;
+  hipLaunchKernelGGL(spectral_weights_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, (int)kind,
+                     (int)ns, (int)k, (const double2*)lam, s, W);
+  PQA_HIP(hipGetLastError());
+  const bool wide = ncol >= 48;
+  const int tbw = wide ? 4 * kTB : kTB;
+  const dim3 grid((unsigned)((ns + kTT - 1) / kTT), (unsigned)((ncol + tbw - 1) / tbw), (unsigned)nrow);
+  if (grid.y > 65535) return fail(PYQED_EINVAL, "spectral_sum: too many columns");
+  const bool tfast = so_t == 1;
+  const double2 *Wc = (const double2*)W, *Ac = (const double2*)A, *Bc = (const double2*)B;
+  double2* O = (double2*)out;
+  if (wide && tfast)
+    hipLaunchKernelGGL((spectral_contract_kernel<4, true>), grid, dim3(kThreads), 0, st, (int)ns, (int)k,
+                       (int)ncol, Wc, Ac, Bc, O, so_t, so_a, so_b);
+  else if (wide)
+    hipLaunchKernelGGL((spectral_contract_kernel<4, false>), grid, dim3(kThreads), 0, st, (int)ns, (int)k,
+                       (int)ncol, Wc, Ac, Bc, O, so_t, so_a, so_b);
+  else if (tfast)
+    hipLaunchKernelGGL((spectral_contract_kernel<1, true>), grid, dim3(kThreads), 0, st, (int)ns, (int)k,
+                       (int)ncol, Wc, Ac, Bc, O, so_t, so_a, so_b);
+  else
+    hipLaunchKernelGGL((spectral_contract_kernel<1, false>), grid, dim3(kThreads), 0, st, (int)ns, (int)k,
+                       (int)ncol, Wc, Ac, Bc, O, so_t, so_a, so_b);
+  PQA_HIP(hipGetLastError());
+  return PYQED_OK;
+}

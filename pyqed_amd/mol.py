@@ -15,7 +15,12 @@ class Result:
         self.rho0 = rho0
         self.psi0 = psi0
         self.nout = nout
-        self.times = t0 + np.arange(Nt // nout + 1) * dt * nout
+        if Nt is None and times is not None:
+            # an explicit time grid (the eigen-decomposition solvers pass one); the reference's
+            # Result(times=...) raises on Nt // nout with Nt None
+            self.times = np.asarray(times)
+        else:
+            self.times = t0 + np.arange(Nt // nout + 1) * dt * nout
 
     def expect(self):
         return self.observables

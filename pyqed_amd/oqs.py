@@ -15,10 +15,12 @@ Documented differences from the reference:
   * rholist holds dense numpy arrays.
 """
 import numpy as np
+import scipy.linalg
 
 from . import _lib
 from .liouville import dag, lindblad_generator, op2sop, propagate
 from .mol import Result
+from .spectral import TIME, spectral_sum
 
 
 class LindbladSolver:
@@ -174,7 +176,13 @@ class RedfieldSolver:
         self.spectra = spectra
         self.evecs = None
         self.dim = H.shape[0]
+        self.U = None
+        self.G = None
         self.e_ops = e_ops
+
+    def idm(self, sp=True):
+        """vec of the identity operator (oqs.py:45-49), dense for either value of sp."""
+        return np.eye(self.dim).reshape(-1)
 
     def configure(self, H, c_ops, e_ops):
         self.c_ops = c_ops
@@ -188,6 +196,84 @@ class RedfieldSolver:
         self.R = R
         self.evecs = evecs
         return R, evecs
+
+    def propagator(self, t, method='SOS'):
+        """Liouville-space propagator U(t) = exp(R t) on the grid t, shape [N, N, len(t)]
+        (oqs.py:160-209).  'SOS' / 'eseries': R = U1 diag(lambda) U1^-1 on the host, the expansion
+        on the GPU spectral-sum kernel.  'EOM': the reference's RK4 of dU/dt = R U from U = 1 with
+        dt = t[1] - t[0] (phys.expm, phys.py:2049-2091), every column of U one instance of the
+        Liouville kernel; entry k is U after k steps.  Sets self.U and self.G = -i U."""
+        if self.R is None:
+            raise TypeError('Redfield tensor is not computed. Please call redfield_tensor()')
+        t = np.asarray(t, np.float64).reshape(-1)
+        R = np.asarray(self.R, np.complex128)
+        N = R.shape[0]
+        if method == 'EOM':
+            print('Computing the propagator. Please make sure that the ground-state energy is 0.')
+            U = np.zeros((N, N, len(t)), np.complex128)
+            U[:, :, 0] = np.eye(N)
+            if len(t) > 1:
+                d = self.dim
+                cols = np.eye(N, dtype=np.complex128).reshape(N, d, d)
+                _, states, _ = propagate(R[None], cols, t[1] - t[0], len(t) - 1, None, ())
+                U[:, :, 1:] = np.transpose(states.reshape(N, len(t) - 1, N), (2, 0, 1))
+            self.U = U
+        elif method in ['eseries', 'SOS']:
+            evals, U1 = scipy.linalg.eig(R)
+            U2 = scipy.linalg.inv(U1)
+            self.U = spectral_sum(TIME, U1, evals, U2, t, layout="abt")
+        else:
+            raise ValueError("method must be 'EOM', 'SOS' or 'eseries'")
+        self.G = -1j * self.U
+        return self.U
+
+    def expect(self, rho0, e_ops):
+        """Observables from the stored propagator (oqs.py:211-242): rho0 and e_ops to the eigenbasis,
+        out[k, j] = sum(vec(e_j) * U[:, :, k] vec(rho0)) with vec row-major, as the reference."""
+        ev = self.evecs
+        U = self.U
+        rho0_eb = (dag(ev) @ np.asarray(rho0) @ ev).reshape(-1)
+        e_eb = [dag(ev) @ np.asarray(e) @ ev for e in e_ops]
+        rho = np.tensordot(U, rho0_eb, axes=([1], [0]))
+        out = np.zeros((U.shape[-1], len(e_ops)), dtype=complex)
+        for j, e in enumerate(e_eb):
+            out[:, j] = e.reshape(-1).dot(rho)
+        return out
+
+    def correlation_2op_1t(self, rho0, a, b, tau):
+        """<<I| a G(tau) b |rho0>> with Liouville-space operators a, b (oqs.py:244-268).  The
+        reference's sparse column idm makes this product raise; the row-vector contraction it
+        intends is computed here."""
+        if self.G is None:
+            self.propagator(tau)
+        G = self.G
+        idm = self.idm()
+        rho0 = np.asarray(rho0)
+        if rho0.ndim == 2:
+            rho0 = rho0.reshape(-1)
+        a = np.asarray(a.toarray() if hasattr(a, "toarray") else a)
+        b = np.asarray(b.toarray() if hasattr(b, "toarray") else b)
+        return idm @ (a @ np.tensordot(G, b @ rho0, axes=([1], [0])))
+
+    def correlation_4op_3t(self, rho0, oplist, signature, tau):
+        """<<I| A G(tau3) B G(tau2) C G(tau1) D |rho0>> as [tau3, tau2, tau1] (oqs.py:270-360), the
+        operators promoted with signature characters from 'lr+-' (or 'c', 'a')."""
+        from .superoperator import operator_to_superoperator
+        if len(oplist) != 4:
+            raise ValueError('Number of operators is not 4.')
+        a, b, c, d = (operator_to_superoperator(op, sg) for op, sg in zip(oplist, signature))
+        if self.G is None:
+            self.propagator(tau)
+        G = self.G
+        idm = self.idm(sp=False)
+        rho0 = np.asarray(rho0.toarray() if hasattr(rho0, "toarray") else rho0)
+        rho = d @ rho0.reshape(-1)
+        tmp = np.tensordot(G, rho, axes=((1), (0)))
+        tmp = c @ tmp
+        tmp = np.tensordot(G, tmp, axes=([1], [0]))
+        tmp = np.tensordot(b, tmp, axes=([1], [0]))
+        tmp = np.tensordot(G, tmp, axes=([1], [0]))
+        return np.einsum('a, ab, bijk -> ijk', idm, a, tmp)
 
     def evolve(self, rho0, dt, Nt, evecs=None, e_ops=[], store_states=False, t0=0, nout=1):
         """RK4 on vec(rho) in the eigenbasis (oqs.py:57-81, 364-459): observables [Nt, n_e] in the
