@@ -169,7 +169,9 @@ class DEOMSolver:
         C(wx, wy) = Tr_0[ d G(wx) c e^{L T} b G(wy) a rho0 ],  G(w) = (-L - i w)^-1,
         with operator x acting as lcr[.] ('l' x., 'r' .x, 'c' x. + .x; a uses lcr[0], d lcr[3]),
         evaluated in the eigenbasis of the dense generator L; with if_full=False only eigenvalues with
-        cut_off_min*min(Re) < Re < cut_off_max*max(Re) are kept."""
+        cut_off_min*min(Re) < Re < cut_off_max*max(Re) are kept.  The eigen-decomposition runs on the
+        host, as in the reference; the frequency grid, the reference's double loop, runs on the GPU
+        spectral-sum kernel."""
         import os
         import scipy.linalg as la
         from .generator import system_action
@@ -196,13 +198,12 @@ class DEOMSolver:
         left = (act["d"] @ V)[:n * n]                                              # d V (ADO 0 rows)
         mid = ((Vi @ act["c"] @ V) * np.exp(lam * T)[None, :]) @ (Vi @ act["b"] @ V)  # c e^{LT} b
         right = Vi @ (act["a"] @ rho)                                              # a rho
-        out = np.zeros((len(w_x), len(w_y)), complex)
-        for i, wx in enumerate(w_x):
-            A = left * (1.0 / (-lam - 1j * wx))[None, :]
-            for j, wy in enumerate(w_y):
-                v = A @ (mid @ ((1.0 / (-lam - 1j * wy)) * right))
-                out[i, j] = np.trace(v.reshape(n, n))
-        return out
+        trace_row = left.reshape(n, n, -1)[np.arange(n), np.arange(n)].sum(axis=0)  # Tr over ADO 0
+        # the (w_x, w_y) grid as two resolvent spectral sums on the GPU, 1/(-lam - i w) = -1/(lam + i w):
+        #   P[j, m] = sum_k mid[m, k] g_k(w_y[j]) right[k];  C[i, j] = sum_m trace_row[m] g_m(w_x[i]) P[j, m]
+        from ..spectral import FREQ, spectral_sum
+        P = spectral_sum(FREQ, mid, lam, right[:, None], w_y)[:, :, 0]
+        return spectral_sum(FREQ, trace_row[None, :], lam, P.T, w_x)[:, 0, :]
 
     @property
     def ddos(self):

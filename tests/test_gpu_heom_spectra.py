@@ -1,0 +1,66 @@
+"""DEOMSolver.correlation_4op_3t (deom.py:1127-1210): the eigen-decomposition correlation, whose
+frequency grid runs on the GPU spectral-sum kernel, against dense linear solves of the resolvents and
+expm of the generator (independent of the eigen-decomposition).  Tolerance 1e-9 relative."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from conftest import bath_of, golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(g, L):
+    from pyqed_amd.heom.deom import Bath, DEOMSolver
+    return DEOMSolver(g["H"], None, Bath.from_arrays(*bath_of(g)), list(g["Q"]), None, None, None, L)
+
+
+def test_correlation_4op_3t_against_linear_solves():
+    from pyqed_amd.heom.generator import system_action
+    g = golden("traj_c1.npz")
+    s = _solver(g, 3)
+    sx = np.array([[0, 1], [1, 0]], complex)
+    sz = np.diag([1.0, -1.0]).astype(complex)
+    rho0 = g["rho0"]
+    wx, wy = np.array([-0.5, 0.3, 0.9]), np.linspace(-1.0, 1.5, 70)
+    T = 0.8
+    for lcr in ("llll", "lrcl"):
+        C = s.correlation_4op_3t(sx, sz, sx, sz, rho0, T, wx, wy, lcr=lcr)
+        G = s.propgator
+        n = G.shape[0]
+        acts = [system_action(op, s.nmax, c).toarray() for op, c in zip((sx, sz, sx, sz), lcr)]
+        v0 = np.zeros(n, complex)
+        v0[:4] = rho0.reshape(-1)
+        E = sla.expm(G * T)
+        for i, a in enumerate(wx):
+            for j, b in enumerate(wy):
+                x = np.linalg.solve(-G - 1j * b * np.eye(n), acts[0] @ v0)
+                x = acts[2] @ (E @ (acts[1] @ x))
+                x = acts[3] @ np.linalg.solve(-G - 1j * a * np.eye(n), x)
+                assert abs(np.trace(x[:4].reshape(2, 2)) - C[i, j]) < 1e-9 * max(1.0, abs(C[i, j]))
+
+
+def test_correlation_4op_3t_cut_off_keeps_the_selected_modes():
+    g = golden("traj_c1.npz")
+    s = _solver(g, 3)
+    sx = np.array([[0, 1], [1, 0]], complex)
+    sz = np.diag([1.0, -1.0]).astype(complex)
+    wx, wy = np.array([0.2, 0.6]), np.array([-0.3, 0.4])
+    full = s.correlation_4op_3t(sx, sz, sx, sz, g["rho0"], 0.5, wx, wy)
+    cut = s.correlation_4op_3t(sx, sz, sx, sz, g["rho0"], 0.5, wx, wy, if_full=False)
+    # the reference's selection: cut_off_min * min(Re) < Re < cut_off_max * max(Re), evaluated directly
+    from pyqed_amd.heom.generator import system_action
+    lam, V, Vi = s._delta, s._V, s._Vinv
+    re = np.real(lam)
+    keep = (re > np.min(re) * 0.5) & (re < np.max(re) * 1.1)
+    acts = [system_action(op, s.nmax, "l").toarray() for op in (sx, sz, sx, sz)]
+    rho = np.zeros(V.shape[0], complex)
+    rho[:4] = g["rho0"].reshape(-1)
+    V1, Vi1, l1 = V[:, keep], Vi[keep], lam[keep]
+    for i, a in enumerate(wx):
+        for j, b in enumerate(wy):
+            x = (1 / (-l1 - 1j * b)) * (Vi1 @ (acts[0] @ rho))
+            x = ((Vi1 @ acts[2] @ V1) * np.exp(l1 * 0.5)) @ ((Vi1 @ acts[1] @ V1) @ x)
+            x = (acts[3] @ V1)[:4] @ ((1 / (-l1 - 1j * a)) * x)
+            assert abs(np.trace(x.reshape(2, 2)) - cut[i, j]) < 1e-9 * max(1.0, abs(cut[i, j]))
+    assert full.shape == cut.shape == (2, 2)
