@@ -16,6 +16,9 @@ Two routes:
     normalisation).
   * drude(...) evaluates the Drude-Lorentz + Pade decomposition in closed form
     (physical normalisation), with no sympy: the route for large batches.
+  * decompose_spectrum_prony(...) and its helpers follow deom.py:21-66, 310-542: C(t) sampled
+    from the Pade decomposition and fitted by Prony's method (Hankel eigenvectors, polynomial roots,
+    least-squares amplitudes).  Host-only preprocessing, run once per bath.
 """
 import math
 
@@ -165,6 +168,171 @@ def decompose_spectrum_pade(spe, w_sp, beta, npsd, pade=1, bose_fermi=1, monic=F
         etaa.append(abs(e))
     return (np.array(etal, np.complex128), np.array(etar, np.complex128), np.array(etaa, np.float64),
             np.array(expn, np.complex128))
+
+
+# ----------------------------------------------------------------------------- Prony route
+def spectrum_exp(w, res, expn, etal, sigma=-1):
+    """res += sum_k etal_k / (expn_k + sigma i w), in place and term by term (deom.py:21-24)."""
+    for e, x in zip(etal, expn):
+        res += e / (x + sigma * 1j * w)
+
+
+def fit_t(t, res, expn, etal):
+    """res += sum_k etal_k exp(-expn_k t), in place and term by term (deom.py:62-66)."""
+    for e, x in zip(etal, expn):
+        res += e * np.exp(-x * t)
+    return res
+
+
+def sort_symmetry(etal, expn, if_sqrt=True):
+    """Order exponentials by decreasing |Im expn| and pair the complex ones (deom.py:37-59): etar of a
+    pair is the conjugate of its partner's etal, etaa = |etal| (square-rooted with if_sqrt).  As in
+    the reference, etar starts as etal permuted a second time and every entry is then overwritten."""
+    order = np.argsort(np.abs(np.imag(expn)))[::-1]
+    mags = np.sort(np.abs(np.imag(expn)))[::-1]
+    expn = expn[order]
+    etal = etal[order]
+    etar = etal[order]
+    ncc = len(np.where(expn[mags > 1e-10])[0])
+    etaa = np.zeros(len(etal), dtype=float)
+    for i in range(0, ncc, 2):
+        etar[i], etar[i + 1] = np.conj(etal[i + 1]), np.conj(etal[i])
+        etaa[i], etaa[i + 1] = np.abs(etal[i]), np.abs(etal[i + 1])
+    for i in range(ncc, len(expn)):
+        etar[i] = np.conj(etal[i])
+        etaa[i] = np.abs(etal[i])
+    if if_sqrt:
+        etaa = np.sqrt(etaa)
+    return etal, etar, etaa, expn
+
+
+def decompose_spectrum_pade_real(spe, w_sp):
+    """Exponentials of one part of a sympy spectral density without the Bose factor
+    (deom.py:310-365): its imaginary part if nonzero, else its real part.  etal follows the sorted
+    poles (complex pairs by decreasing |Im|, then the rest); expn is returned unsorted, as in the
+    reference."""
+    import sympy as sp
+
+    re_part, im_part = sp.cancel(spe).as_real_imag()
+    part = re_part if im_part == 0 else im_part
+    numer, denom = sp.cancel(sp.factor(part)).as_numer_denom()
+    roots = [complex(r) for r in sp.nroots(denom)]
+    allp = np.array(roots, np.complex128)
+    expn = np.array([1j * r for r in roots if r.imag < 0], np.complex128)
+    order = np.argsort(np.abs(np.imag(expn)))[::-1]
+    mags = np.sort(np.abs(np.imag(expn)))[::-1]
+    ccs = expn[order[mags != 0]]
+    reals = expn[order[mags == 0]]
+    num_f = sp.lambdify(w_sp, numer, "numpy")
+
+    def coefficient(ex):
+        p = -1j * ex
+        others = allp[np.abs(allp - p) > 1e-14]
+        return complex(-1j * complex(num_f(p)) / complex(np.prod(p - others)))
+
+    etal, etar, etaa = [], [], []
+    for ii in range(0, len(ccs), 2):
+        e0, e1 = coefficient(ccs[ii]), coefficient(ccs[ii + 1])
+        etal += [e0, e1]
+        etar += [np.conj(e1), np.conj(e0)]
+        etaa += [np.sqrt(abs(e0) * abs(np.conj(e1))), np.sqrt(abs(e1) * abs(np.conj(e0)))]
+    for ex in reals:
+        e = coefficient(ex)
+        etal.append(e)
+        etar.append(np.conj(e))
+        etaa.append(np.sqrt(abs(e) * abs(np.conj(e))))
+    return np.array(etal), np.array(etar), np.array(etaa), np.array(expn)
+
+
+def decompose_spectrum_pade_imag(spe, w_sp):
+    """The reference's second copy of decompose_spectrum_pade_real (deom.py:368-424), identical in
+    effect."""
+    return decompose_spectrum_pade_real(spe, w_sp)
+
+
+def prony_find_gamma(h, n_sample, nind):
+    """Prony roots (deom.py:427-444): eigen-decompose the n_sample x n_sample Hankel matrix of h
+    (scipy.linalg.eig), rotate each eigenvector by exp(-i arg(sigma)/2), sort by |sigma| descending,
+    take the roots of the polynomial whose coefficients are the entries of vector number nind
+    (highest degree last) and keep the nind roots of smallest modulus."""
+    import scipy.linalg as la
+
+    mat_h = np.zeros((n_sample, n_sample))
+    for i in range(n_sample):
+        mat_h[i, :] = h[i:n_sample + i]
+    sing_vs, Q = la.eig(mat_h)
+    phase_mat = np.diag([np.exp(-1j * np.angle(sv) / 2.0) for sv in sing_vs])
+    vs = np.array([np.abs(sv) for sv in sing_vs])
+    Qp = np.dot(Q, phase_mat)
+    order = np.argsort(vs)[::-1]
+    Qp = Qp[:, order]
+    print(nind)
+    gamma = np.roots(Qp[:, nind][::-1])
+    return gamma[np.argsort(np.abs(gamma))[:nind]]
+
+
+def prony_fitting(h, t, nind, scale, n, gamma_real=None, gamma_imag=None):
+    """Least-squares amplitudes for Prony roots (deom.py:447-504): gamma from prony_find_gamma on Re h
+    (nind an int) or separately on Re h and Im h (nind = [n_real, n_imag]) unless given; the normal
+    equations of the stacked real system; expn = -2 n log(gamma) / scale; then sort_symmetry."""
+    import scipy.linalg as la
+
+    if isinstance(nind, list):
+        gamma_real = prony_find_gamma(np.real(h), n, nind[0]) if gamma_real is None else np.array(gamma_real)
+        gamma_imag = prony_find_gamma(np.imag(h), n, nind[1]) if gamma_imag is None else np.array(gamma_imag)
+        gamma = np.append(gamma_real, gamma_imag)
+        n_row = nind[0] + nind[1]
+    else:
+        gamma = prony_find_gamma(np.real(h), n, nind)
+        n_row = nind
+    t_new = 2 * n * np.log(gamma)
+    n_col = len(t)
+    gamma_m = np.zeros((2 * n_col, 2 * n_row), dtype=float)
+    for i in range(n_row):
+        for j in range(n_col):
+            gj = gamma[i] ** j
+            gamma_m[j, i] = np.real(gj)
+            gamma_m[n_col + j, n_row + i] = np.real(gj)
+            gamma_m[j, n_row + i] = -np.imag(gj)
+            gamma_m[n_col + j, i] = np.imag(gj)
+    h_m = np.append(np.real(h), np.imag(h))
+    omega = (la.inv(gamma_m.T @ gamma_m) @ gamma_m.T @ h_m).reshape(2, n_row)
+    etal_p = omega[0, :] + 1.j * omega[1, :]
+    expn_p = -t_new / scale
+    return sort_symmetry(etal_p, expn_p)
+
+
+def decompose_spectrum_prony(spe, w_sp, beta, nind, scale=250000, n=1250, npsd=10, bose_fermi=1):
+    """Prony decomposition of a sympy spectral density (deom.py:507-542): sample C(t) from its Pade
+    decomposition (npsd poles) at scale * linspace(0, 1, 2n + 1) and fit nind exponentials.  With
+    nind = ['a', k] (or [k, 'a']) the real (imaginary) part keeps its analytic exponentials; the
+    reference ends the program for the Bose real-part case and the Fermi imaginary-part case, and so
+    does this function (SystemExit)."""
+    etal_pade, _, _, expn_pade = decompose_spectrum_pade(spe, w_sp, beta, npsd, bose_fermi=bose_fermi)
+    t = np.linspace(0, 1, 2 * n + 1)
+    res_t = np.zeros(len(t), dtype=complex)
+    fit_t(scale * t, res_t, expn_pade, etal_pade)
+    print("check the sample points")
+    print(res_t[:10])
+    print(res_t[-10:])
+    if isinstance(nind, list):
+        if nind[0] == 'a':
+            _, _, _, expn_real = decompose_spectrum_pade_real(spe, w_sp)
+            gamma_real = np.exp(-expn_real * scale / (2 * n))
+            nind[0] = len(gamma_real)
+            if bose_fermi == 1:
+                print("For the bose case, C(t) have the analytical imag part")
+                raise SystemExit
+            return prony_fitting(res_t, t, nind, scale, n, gamma_real=gamma_real)
+        elif nind[1] == 'a':
+            _, _, _, expn_imag = decompose_spectrum_pade_imag(spe, w_sp)
+            gamma_imag = np.exp(-expn_imag * scale / (2 * n))
+            nind[1] = len(gamma_imag)
+            if bose_fermi == 2:
+                print("For the fermi case, C(t) have the analytical real part")
+                raise SystemExit
+            return prony_fitting(res_t, t, nind, scale, n, gamma_imag=gamma_imag)
+    return prony_fitting(res_t, t, nind, scale, n)
 
 
 def single_oscillator(omega, w_sp, beta, nind):

@@ -16,12 +16,16 @@ right-hand side with FP64 state), save_every (keep every n-th rho_0).
 """
 import numpy as np
 
-from .bath import (Bath, decompose_spectrum_pade, drude, function_bose, matsubara_approximation_distribution,
-                   pade_approximation_distribution, single_oscillator)
+from .bath import (Bath, decompose_spectrum_pade, decompose_spectrum_pade_imag, decompose_spectrum_pade_real,
+                   decompose_spectrum_prony, drude, fit_t, function_bose, matsubara_approximation_distribution,
+                   pade_approximation_distribution, prony_find_gamma, prony_fitting, single_oscillator,
+                   sort_symmetry, spectrum_exp)
 from .engine import HeomEngine, field_table
 
 __all__ = ["Bath", "DEOMSolver", "decompose_spectrum_pade", "pade_approximation_distribution",
-           "matsubara_approximation_distribution", "function_bose", "single_oscillator", "drude"]
+           "matsubara_approximation_distribution", "function_bose", "single_oscillator", "drude",
+           "decompose_spectrum_prony", "prony_fitting", "prony_find_gamma", "decompose_spectrum_pade_real",
+           "decompose_spectrum_pade_imag", "sort_symmetry", "fit_t", "spectrum_exp"]
 
 
 class DEOMSolver:

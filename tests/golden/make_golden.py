@@ -17,6 +17,8 @@ Fixtures written (all under tests/golden/):
   tables_*.npz   keys / plus / minus tables from gen_keys, hash_plus,
                  hash_minus (deom.py:555-638) for small (K, L), and SHA-256
                  digests of the tables for the C2 / C5 sizes.
+  prony.npz      decompose_spectrum_prony, decompose_spectrum_pade_real / _imag, sort_symmetry,
+                 fit_t and spectrum_exp outputs (deom.py:21-66, 310-542).
   traj_*.npz     DEOMSolver.run trajectories (deom.py:1072-1114): rho_0(t)
                  for the spin-boson case (1000 steps), the 7-site exciton
                  (20 steps), the sigma_x-coupled two-level example, a driven
@@ -276,9 +278,55 @@ def traj_c5_onestep(ref):
          **{"bath_" + k: v for k, v in bath_arrays(bath).items()})
 
 
+# --------------------------------------------------------------------------- Prony route
+def gen_prony(ref):
+    """decompose_spectrum_prony (deom.py:507-542) on a Drude spectrum (nind = 3, one fit of Re C) and a
+    Brownian spectrum (nind = [2, 2], separate fits of Re C and Im C), over t in [0, 20] sampled at
+    2n + 1 = 401 points; decompose_spectrum_pade_real / _imag (deom.py:310-424) of both spectra;
+    sort_symmetry, fit_t and spectrum_exp (deom.py:21-66) on fixed small arrays."""
+    import sympy as sp
+    out = {}
+    spe, w = drude_sympy(0.2, 1.0)
+    r = ref.decompose_spectrum_prony(spe, w, 1.0, 3, scale=20.0, n=200, npsd=10)
+    out.update({f"drude_{k}": np.asarray(v) for k, v in zip(("etal", "etar", "etaa", "expn"), r)})
+    for tag, f in (("real", ref.decompose_spectrum_pade_real), ("imag", ref.decompose_spectrum_pade_imag)):
+        r = f(spe, w)
+        out.update({f"drude_pade_{tag}_{k}": np.asarray(v) for k, v in zip(("etal", "etar", "etaa", "expn"), r)})
+    wsym = sp.symbols(r"\omega", real=True)
+    lam, w0, g = 0.5, 1.0, 0.6
+    spe_b = 2 * lam * w0 ** 2 * g * wsym / ((w0 ** 2 - wsym ** 2) ** 2 + g ** 2 * wsym ** 2)
+    r = ref.decompose_spectrum_prony(spe_b, wsym, 1.0, [2, 2], scale=20.0, n=200, npsd=10)
+    out.update({f"brownian_{k}": np.asarray(v) for k, v in zip(("etal", "etar", "etaa", "expn"), r)})
+    # the samples both fits start from, as decompose_spectrum_prony forms them (deom.py:519-523):
+    # the separate real / imaginary fits are ill-conditioned, so the fitting stage is pinned on
+    # these exact inputs
+    for tag, sp_e, sym in (("drude", spe, w), ("brownian", spe_b, wsym)):
+        etal_p, _, _, expn_p = ref.decompose_spectrum_pade(sp_e, sym, 1.0, 10)
+        tt = np.linspace(0, 1, 401)
+        res_t = np.zeros(401, dtype=complex)
+        ref.fit_t(20.0 * tt, res_t, expn_p, etal_p)
+        out[f"{tag}_samples"] = res_t
+    r = ref.decompose_spectrum_pade_real(spe_b, wsym)
+    out.update({f"brownian_pade_real_{k}": np.asarray(v) for k, v in zip(("etal", "etar", "etaa", "expn"), r)})
+    etal = np.array([0.3 + 0.1j, -0.2j, 0.5, 0.1 - 0.4j, 0.25 + 0.05j])
+    expn = np.array([1.0 + 2.0j, 0.5, 1.0 - 2.0j, 2.0 + 0.5j, 2.0 - 0.5j])
+    r = ref.sort_symmetry(etal.copy(), expn.copy())
+    out.update({f"sort_{k}": np.asarray(v) for k, v in zip(("etal", "etar", "etaa", "expn"), r)})
+    out["sort_in_etal"], out["sort_in_expn"] = etal, expn
+    t = np.linspace(0.0, 3.0, 7)
+    res = np.zeros(7, complex)
+    ref.fit_t(t, res, expn, etal)
+    out["fit_t"], out["fit_t_t"] = res, t
+    res = np.zeros(7, complex)
+    ref.spectrum_exp(t, res, expn, etal)
+    out["spectrum_exp"] = res
+    save("prony.npz", **out)
+    return out
+
+
 JOBS = {
     "baths": gen_baths, "tables": gen_tables, "c1": traj_c1, "sx": traj_example_sx,
-    "driven": traj_driven, "c2": traj_c2, "c3": traj_c3_small, "c5": traj_c5_onestep,
+    "driven": traj_driven, "c2": traj_c2, "c3": traj_c3_small, "c5": traj_c5_onestep, "prony": gen_prony,
 }
 
 if __name__ == "__main__":
