@@ -353,12 +353,68 @@ def bench_spectral(args, torch):
     print(json.dumps(res), flush=True)
 
 
+KRYLOV_METRIC = "two-frequency HEOM correlation grid points/s (Krylov route, FP64), 1 GPU"
+
+
+def bench_krylov(args, torch):
+    """DEOMSolver.correlation_4op_3t(method='krylov') for the 7-site exciton (K = 14, depth 3: 680 ADOs,
+    a generator of order 33320, beyond a dense eigen-decomposition), 8 x 8 frequency grid."""
+    from pyqed_amd import workloads as W
+    from pyqed_amd.heom.deom import Bath, DEOMSolver
+    from pyqed_amd.heom.bath import drude
+    from pyqed_amd.heom.engine import HeomEngine
+    w = W.exciton_c2()
+    d = w["H"].shape[0]
+    e = drude(w["lam"], w["gam"], w["beta"], w["npsd"])
+    bath = Bath.from_arrays(*(np.tile(a, d) for a in e), np.repeat(np.arange(d), len(e[3])))
+    s = DEOMSolver(w["H"], None, bath, list(w["Q"]), None, None, None, 3)
+    mu = np.zeros((d, d), complex)
+    mu[0, 1:] = mu[1:, 0] = 1.0
+    wx = np.linspace(-0.002, 0.002, 8)
+    wy = np.linspace(-0.002, 0.002, 8)
+    s.correlation_4op_3t(mu, mu, mu, mu, w["rho0"], 500.0, wx[:1], wy[:1], lcr="lrcl", method="krylov")  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    C = s.correlation_4op_3t(mu, mu, mu, mu, w["rho0"], 500.0, wx, wy, lcr="lrcl", method="krylov")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # the product L x itself: the FP64 right-hand-side kernel on one full-layout vector
+    eng = HeomEngine([s._instance(w["rho0"])], packed=False)
+    M = int(eng.nmax[0]) * d * d
+    x = torch.randn(((M + 15) // 16) * 16, dtype=torch.complex128, device=eng.device)
+    for _ in range(3):
+        eng.rhs(x)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    st = torch.cuda.current_stream()
+    ev[0].record(st)
+    for _ in range(100):
+        eng.rhs(x)
+    ev[1].record(st)
+    torch.cuda.synchronize()
+    mv_ms = ev[0].elapsed_time(ev[1]) / 100
+    eng.close()
+    byts = 32.0 * M  # read x, write L x, FP64 complex; neighbour re-reads counted once
+    res = {
+        "metric": KRYLOV_METRIC, "value": C.size / wall, "unit": "grid points/s", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (7-site exciton chain, Drude baths; transition-dipole operators)",
+        "config": {"workload": f"7-site exciton, depth 3, generator order {M}, 8 x 8 grid, T = 500 au, tol 1e-10"},
+        "roofline": {"bound": "hbm", "kernel": "L x (pyqed_heom_rhs, FP64 generic stage kernel)",
+                     "launch_ms": mv_ms, "bytes_per_launch": byts, "achieved": byts / (mv_ms / 1e3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": byts / (mv_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None},
+        "cpu_baseline": None,
+        "note": "no CPU baseline: the reference's route needs la.eig of the dense generator (order 33320)",
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; 2000 for --workload legacy)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c5", "legacy", "spectral"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c5", "legacy", "spectral", "krylov"])
     ap.add_argument("--n-inst", type=int, default=None, help="instances (default 4096; 65536 for legacy)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--state", default="auto", choices=["auto", "full"],
@@ -403,6 +459,9 @@ def main():
         return
     if args.workload == "spectral":
         bench_spectral(args, torch)
+        return
+    if args.workload == "krylov":
+        bench_krylov(args, torch)
         return
 
     insts, dt, desc = instances(args.workload, args.n_inst)

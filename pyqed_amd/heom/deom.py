@@ -168,14 +168,26 @@ class DEOMSolver:
         return self.propgator
 
     def correlation_4op_3t(self, operator_a, operator_b, operator_c, operator_d, rho0, T, w_x, w_y, if_full=True,
-                           cut_off_min=0.5, cut_off_max=1.1, if_load=False, if_save=False, lcr="llll"):
+                           cut_off_min=0.5, cut_off_max=1.1, if_load=False, if_save=False, lcr="llll", *,
+                           method="eig", tol=1e-10, krylov_dim=400):
         """Two-frequency four-point correlation (deom.py:1127-1210):
         C(wx, wy) = Tr_0[ d G(wx) c e^{L T} b G(wy) a rho0 ],  G(w) = (-L - i w)^-1,
         with operator x acting as lcr[.] ('l' x., 'r' .x, 'c' x. + .x; a uses lcr[0], d lcr[3]),
         evaluated in the eigenbasis of the dense generator L; with if_full=False only eigenvalues with
         cut_off_min*min(Re) < Re < cut_off_max*max(Re) are kept.  The eigen-decomposition runs on the
         host, as in the reference; the frequency grid, the reference's double loop, runs on the GPU
-        spectral-sum kernel."""
+        spectral-sum kernel.
+
+        method="krylov" avoids the dense generator altogether: every product with L is the GPU
+        right-hand-side kernel, the resolvents of all frequencies share one Arnoldi basis per starting
+        vector and e^{L T} is a Krylov exponential (pyqed_amd.heom.krylov), each basis grown until its
+        residual is below tol relative to its starting vector (at most krylov_dim vectors).  It
+        evaluates the same expression as if_full=True, for hierarchies too large for la.eig."""
+        if method == "krylov":
+            return self._correlation_krylov((operator_a, operator_b, operator_c, operator_d), rho0, T, w_x, w_y,
+                                            lcr, tol, krylov_dim)
+        if method != "eig":
+            raise ValueError("method must be 'eig' or 'krylov'")
         import os
         import scipy.linalg as la
         from .generator import system_action
@@ -208,6 +220,55 @@ class DEOMSolver:
         from ..spectral import FREQ, spectral_sum
         P = spectral_sum(FREQ, mid, lam, right[:, None], w_y)[:, :, 0]
         return spectral_sum(FREQ, trace_row[None, :], lam, P.T, w_x)[:, 0, :]
+
+    def _correlation_krylov(self, ops, rho0, T, w_x, w_y, lcr, tol, krylov_dim):
+        from .. import _lib
+        from .krylov import correlation_grid
+        torch = _lib.require_gpu()
+        self.check_()
+        self.init_()
+        eng = HeomEngine([self._instance(rho0)], packed=False)
+        try:
+            n, nmax = self.nsys, int(eng.nmax[0])
+            M = nmax * n * n
+            dev = eng.device
+            buf = torch.zeros(((M + 15) // 16) * 16, dtype=torch.complex128, device=dev)
+
+            def matvec(v):
+                buf[:M] = v
+                return eng.rhs(buf)[:M]
+
+            def on_blocks(op, side):
+                """The action of op on a stack of d x d blocks [..., n, n] ('l' A., 'r' .A, 'c' A. + .A)."""
+                A = torch.as_tensor(np.array(op, np.complex128, order="C"), device=dev)
+                if side not in ("l", "r", "c"):
+                    raise ValueError(side)
+
+                def f(X):
+                    if side == "l":
+                        return torch.matmul(A, X)
+                    if side == "r":
+                        return torch.matmul(X, A)
+                    return torch.matmul(A, X) + torch.matmul(X, A)
+                return f
+
+            def on_vector(op, side):
+                g = on_blocks(op, side)
+                return lambda v: g(v.reshape(nmax, n, n)).reshape(-1)
+
+            act = {k: on_vector(op, c) for k, op, c in zip("abc", ops[:3], lcr[:3])}
+            d_blocks = on_blocks(ops[3], lcr[3])
+
+            def trace_d(V):
+                """Tr of d applied to ADO 0, for every row of V."""
+                Z = d_blocks(V[:, :n * n].reshape(V.shape[0], n, n))
+                return torch.diagonal(Z, dim1=-2, dim2=-1).sum(-1).cpu().numpy()
+
+            rho = torch.zeros(M, dtype=torch.complex128, device=dev)
+            rho[:n * n] = torch.as_tensor(np.asarray(rho0, np.complex128).reshape(-1), device=dev)
+            return correlation_grid(matvec, act, trace_d, rho, float(T), w_x, w_y, torch, tol=tol, m_max=krylov_dim)
+        finally:
+            eng.close()
 
     @property
     def ddos(self):

@@ -64,3 +64,47 @@ def test_correlation_4op_3t_cut_off_keeps_the_selected_modes():
             x = (acts[3] @ V1)[:4] @ ((1 / (-l1 - 1j * a)) * x)
             assert abs(np.trace(x.reshape(2, 2)) - cut[i, j]) < 1e-9 * max(1.0, abs(cut[i, j]))
     assert full.shape == cut.shape == (2, 2)
+
+
+@pytest.mark.parametrize("lcr", ["llll", "lrcl"])
+def test_krylov_route_matches_eigen_route(lcr):
+    g = golden("traj_c1.npz")
+    s = _solver(g, 3)
+    sx = np.array([[0, 1], [1, 0]], complex)
+    sz = np.diag([1.0, -1.0]).astype(complex)
+    wx, wy = np.array([-0.5, 0.3, 0.9]), np.linspace(-1.0, 1.5, 9)
+    ref = s.correlation_4op_3t(sx, sz, sx, sz, g["rho0"], 0.8, wx, wy, lcr=lcr)
+    kry = s.correlation_4op_3t(sx, sz, sx, sz, g["rho0"], 0.8, wx, wy, lcr=lcr, method="krylov", tol=1e-12)
+    assert np.abs(kry - ref).max() < 1e-8 * np.abs(ref).max()
+
+
+def test_krylov_route_on_a_hierarchy_without_dense_eigendecomposition():
+    """7-site exciton, depth 2 (120 ADOs, generator of order 5880): the Krylov grid against sparse LU
+    solves of the resolvents and scipy's expm_multiply on the sparse generator."""
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spla
+    from pyqed_amd.heom.generator import heom_generator, system_action
+    g = golden("traj_c2.npz")
+    s = _solver(g, 2)
+    d = g["H"].shape[0]
+    rng = np.random.default_rng(0)
+    mu = np.zeros((d, d), complex)
+    mu[0, 1:] = mu[1:, 0] = rng.uniform(0.5, 1.0, d - 1)
+    # the workload is in atomic units (H ~ 1e-3 hartree): frequencies of the same order, T ~ 24 fs
+    wx, wy = np.array([0.0005, 0.002]), np.array([-0.001, 0.0005, 0.0015])
+    T = 1000.0
+    kry = s.correlation_4op_3t(mu, mu, mu, mu, g["rho0"], T, wx, wy, lcr="lrcl", method="krylov", tol=1e-11)
+    G = heom_generator(g["H"], list(g["Q"]), bath_of(g)[:4], g["bath_mode"], 2, sparse=True).tocsc()
+    n = G.shape[0]
+    assert n == 120 * d * d
+    acts = [system_action(mu, n // (d * d), c) for c in "lrcl"]
+    v0 = np.zeros(n, complex)
+    v0[:d * d] = g["rho0"].reshape(-1)
+    I = sps.identity(n, format="csc")
+    for j, b in enumerate(wy):
+        x = spla.spsolve(-G - 1j * b * I, acts[0] @ v0)
+        x = acts[2] @ spla.expm_multiply(G * T, acts[1] @ x)
+        for i, a in enumerate(wx):
+            z = acts[3] @ spla.spsolve(-G - 1j * a * I, x)
+            ref = np.trace(z[:d * d].reshape(d, d))
+            assert abs(kry[i, j] - ref) < 1e-7 * max(1.0, abs(ref))
