@@ -38,20 +38,41 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ADO-updates/s, FP32-batched vs FP64 plain HEOM, 1 GPU, ≤1e-6 pop. error"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic HBM bytes of one stage launch over a class of F stage-buffer elements (full matrices,
-# complex of ts bytes) and S RK4-state elements (complex double; S = F, or the upper triangles of a
-# packed Hermitian plan):  ts * F * NX[stage] + 16 * S * NS[stage].
-#   stage 0: read y, write ynew, X (packed: also read the full-matrix copy of y)
-#   stage 1, 2: read X, y, ynew; write ynew, X'
-#   stage 3: read X, ynew; write y (packed: also write the full-matrix copy of y)
-# Unpacked mixed precision: 208 B per element per RK4 step; FP64: 256 B.
+# Algorithmic bytes per full-matrix element of the four RK4 stages, for classic RK4 with three
+# registers (y and the accumulator in complex128, the stage input X in complex64 for mixed precision
+# or complex128 for FP64), neighbour re-reads counted once: stage 0 reads y, writes acc and X;
+# stages 1-2 read X, read and write acc, read y, write X; stage 3 reads X and acc, writes y.
+# Mixed 40 + 64 + 64 + 40 = 208 B per element per RK4 step, FP64 48 + 80 + 80 + 48 = 256 B.
+# roofline.achieved prices a launch by these figures times the class's full-matrix elements, the
+# same figure for every state layout and register scheme.
+ALG = {"mixed": (40, 64, 64, 40), "fp64": (48, 80, 80, 48)}
+
+
+def alg_bytes(stage, ts, F):
+    return F * ALG["mixed" if ts == 8 else "fp64"][stage]
+
+
+# Bytes the implementation's own layout moves in one stage launch over a class of F stage-buffer elements (full matrices,
+# complex of ts bytes) and S RK4-state elements (S = F, or the upper triangles of a packed Hermitian
+# plan):  ts * F * NX[stage] + S * SB[stage], with the stage-buffer traffic NX
+#   stage 0: write X (packed: also read the full-matrix FP32/FP64 copy of y, the stage input)
+#   stage 1, 2: read X, write X'
+#   stage 3: read X (packed: also write the full-matrix copy of y)
+# and the RK4-state bytes SB per state element.  FP64 keeps the running new state ynew (complex
+# double): stage 0 reads y, writes ynew; stages 1-2 read y, ynew, write ynew; stage 3 reads ynew,
+# writes y.  Unpacked mixed precision keeps the FP32 increment inc = ynew - y (8 bytes): stage 0
+# reads y (its input), writes inc; stages 1-2 read y, inc, write inc; stage 3 reads y, inc, writes y.
+# Packed mixed precision keeps two FP32 registers A, B and takes y for stages 0-2 from their FP32
+# stage input: stage 0 writes A; stage 1 reads A, writes B; stage 2 reads A, B, writes B; stage 3
+# reads y, B, writes y.  Unpacked: mixed 176 B per element per RK4 step, FP64 256 B.
 NX = {False: (1, 2, 2, 1), True: (2, 2, 2, 2)}
-NS = (2, 3, 3, 2)
+SB = {("fp64", False): (32, 48, 48, 32), ("fp64", True): (32, 48, 48, 32),
+      ("mixed", False): (24, 32, 32, 40), ("mixed", True): (8, 16, 24, 40)}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "heom_proj_kernels_exciton4096_traffic.json")
 
 
 def stage_bytes(stage, ts, F, S, packed):
-    return ts * F * NX[packed][stage] + 16 * S * NS[stage]
+    return ts * F * NX[packed][stage] + S * SB[("mixed" if ts == 8 else "fp64", packed)][stage]
 
 
 def instances(workload, n_inst):
@@ -504,22 +525,27 @@ def main():
     ts = 8 if args.precision == "mixed" else 16
     F = eng.classes()["elems"]
     S = eng.class_state()
-    step_bytes = sum(stage_bytes(s, ts, F[c], S[c], eng.packed) for s in range(4) for c in range(3))
+    step_bytes = sum(alg_bytes(s, ts, F[c]) for s in range(4) for c in range(3))
+    step_layout = sum(stage_bytes(s, ts, F[c], S[c], eng.packed) for s in range(4) for c in range(3))
     t_step = float(np.mean(step_ms)) / 1e3
     roofline = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
+                "bytes_model": "classic three-register RK4, %d B per full-matrix element per step"
+                               % sum(ALG["mixed" if ts == 8 else "fp64"]),
                 "step": {"bytes": step_bytes, "ms": t_step * 1e3, "achieved": step_bytes / t_step / 1e9,
-                         "frac": step_bytes / t_step / 1e9 / HBM_PEAK_GBS}}
+                         "frac": step_bytes / t_step / 1e9 / HBM_PEAK_GBS, "layout_bytes": step_layout}}
     if timing:
         ms, launches = eng.read_timing()
         eng.set_timing(False)
         avg = {k: ms[k] / launches[k] for k in range(12) if launches[k]}
         k = max(avg, key=avg.get)
         s_dom, c_dom = divmod(k, 3)
-        b = stage_bytes(s_dom, ts, F[c_dom], S[c_dom], eng.packed)
+        b = alg_bytes(s_dom, ts, F[c_dom])
+        bl = stage_bytes(s_dom, ts, F[c_dom], S[c_dom], eng.packed)
         name = kernel_name(s_dom, c_dom, args.precision, eng)
         roofline.update({"kernel": name, "stage": s_dom, "class": c_dom, "launch_ms": avg[k],
                          "bytes_per_launch": b, "achieved": b / (avg[k] / 1e3) / 1e9,
                          "frac": b / (avg[k] / 1e3) / 1e9 / HBM_PEAK_GBS,
+                         "layout_bytes_per_launch": bl, "layout_achieved": bl / (avg[k] / 1e3) / 1e9,
                          "launch_ms_all": {f"s{kk // 3}c{kk % 3}": v for kk, v in sorted(avg.items())}})
         roofline["traffic"] = measured_traffic(name, desc, eng.packed)
     else:

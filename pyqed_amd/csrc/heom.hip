@@ -18,7 +18,19 @@
 //   stage 2: k3 = F(X);  ynew += k3 dt/3;      X = y + k3 dt
 //   stage 3: k4 = F(X);  y     = ynew + k4 dt/6
 // Right-hand-side arithmetic runs in T (double for the plain FP64 path,
-// float for the mixed path); y and ynew are always double.
+// float for the mixed path); y is always double.  The FP64 path keeps ynew
+// in double.  The mixed path keeps FP32 increments instead, in the ynew
+// buffer, since an increment is O(dt) and FP32 loses nothing FP64 y keeps:
+//   full-matrix state, one register inc = ynew - y:
+//     stage 0: inc = k1 dt/6;  stages 1-2: inc += k dt/3;  stage 3: y += inc + k4 dt/6
+//   packed Hermitian state, two registers A (first n_state elements of the
+//   ynew buffer) and B (the next n_state), with y for stages 0-2 read from
+//   the FP32 stage input (stage 0's input is y itself; X = y + c k, so
+//   y = X - c k for the k that produced X, which the previous stage wrote):
+//     stage 0: A = k1 dt/6;  X = y + k1 dt/2
+//     stage 1: B = k2 dt/3;  y = X - 3A;  X = y + k2 dt/2
+//     stage 2: B = A + B + k3 dt/3;  y = X - 1.5 B_old;  X = y + k3 dt
+//     stage 3: y = y + B + k4 dt/6   (y read from and written to FP64)
 //
 // Layout in HBM: instances packed one after another; inside an instance ADO
 // a (rank in the reference's combinadic order) occupies elements
@@ -107,6 +119,7 @@ struct StageParams {
   void* xout;
   double2* y;
   double2* ynew;
+  int64_t nstate;     // RK4 state elements (packed mixed runs: offset of the second FP32 register)
   double2* traj;
   int64_t traj_stride;
   int64_t save_slot;  // streamed: slot for this step (-1 = none)
@@ -317,17 +330,99 @@ __device__ __forceinline__ double2 ld_state(const double2* q, const StateRef& s)
   return v;
 }
 
-// Values an element's RK4 update reads, prefetched ahead of the right-hand side.
+// The RK4 register.  FP64 runs keep the running new state in it (ynew = y + dt/6 k1 + dt/3 k2 + ...,
+// complex double).  Mixed-precision runs (DELTA) keep only the increment ynew - y as complex float:
+// stages 0-2 accumulate dt/6 k1 + dt/3 k2 + dt/3 k3 into it and stage 3 forms y + (inc + dt/6 k4).
+// The increments come from FP32 right-hand sides, so FP32 storage loses nothing they carry, and the
+// register moves 8 instead of 16 bytes per access; stage 3 then reads y (16 bytes) as well.
+template <bool DELTA>
 struct Pre {
-  double2 y, an;
+  double2 y;
+  typename std::conditional<DELTA, float2, double2>::type an;
+};
+
+__device__ __forceinline__ float2 ld_delta(const double2* q, const StateRef& s) {
+  float2 v = ((const float2*)q)[s.g];
+  if (s.lower) v.y = -v.y;
+  return v;
+}
+
+template <int STAGE, bool DELTA>
+__device__ __forceinline__ Pre<DELTA> rk4_prefetch(const StageParams& p, const StateRef& s) {
+  Pre<DELTA> v;
+  v.y = (STAGE <= 2 || (DELTA && STAGE == 3)) ? ld_state(p.y, s) : make_double2(0, 0);
+  if constexpr (DELTA) {
+    v.an = (STAGE >= 1 && STAGE <= 3) ? ld_delta(p.ynew, s) : make_float2(0.f, 0.f);
+  } else {
+    v.an = (STAGE >= 1 && STAGE <= 3) ? ld_state(p.ynew, s) : make_double2(0, 0);
+  }
+  return v;
+}
+
+// Packed mixed-precision RK4.  Each upper element has two FP32 registers, A (the low half of the
+// register buffer) and B (the high half, from element nstate on), and stages 0-2 take y from their
+// FP32 stage input, which is in LDS, instead of the FP64 state:
+//   stage 0: A = dt/6 k1;              X   = y + dt/2 k1   (input: y itself)
+//   stage 1: B = dt/3 k2;              X'  = y + dt/2 k2   (input X: y = X - 3 A)
+//   stage 2: B = A + B + dt/3 k3;      X'' = y + dt k3     (input X': y = X' - 3/2 B)
+//   stage 3: y = y + (B + dt/6 k4)     (the FP64 state: read and written)
+// The recovered y carries FP32 accuracy, all that the FP32 stage outputs hold; the increments come
+// from FP32 right-hand sides, and only stage 3 adds them to the FP64 state.
+struct PreM {
+  double2 y;
+  float2 a, b;
 };
 
 template <int STAGE>
-__device__ __forceinline__ Pre rk4_prefetch(const StageParams& p, const StateRef& s) {
-  Pre v;
-  v.y = (STAGE <= 2) ? ld_state(p.y, s) : make_double2(0, 0);
-  v.an = (STAGE >= 1 && STAGE <= 3) ? ld_state(p.ynew, s) : make_double2(0, 0);
+__device__ __forceinline__ PreM rk4_prefetch_m(const StageParams& p, int64_t g) {
+  PreM v;
+  const float2* A = (const float2*)p.ynew;
+  if (STAGE == 1 || STAGE == 2) v.a = A[g];
+  if (STAGE == 2 || STAGE == 3) v.b = A[p.nstate + g];
+  if (STAGE == 3) v.y = p.y[g];
   return v;
+}
+
+// y of an upper element from its FP32 stage input x (stages 0-2).
+template <int STAGE, typename TX>
+__device__ __forceinline__ double2 y_from_input_m(const cpx<TX>& x, const PreM& v) {
+  if (STAGE == 1) return make_double2((double)x.x - 3.0 * (double)v.a.x, (double)x.y - 3.0 * (double)v.a.y);
+  if (STAGE == 2) return make_double2((double)x.x - 1.5 * (double)v.b.x, (double)x.y - 1.5 * (double)v.b.y);
+  return make_double2((double)x.x, (double)x.y);
+}
+
+// The update of upper element g (v.y: y recovered from the input for stages 0-2, the FP64 state for
+// stage 3); returns the element of the stage output.
+template <int STAGE>
+__device__ __forceinline__ double2 rk4_upper_m(const StageParams& p, int64_t g, double rx, double ry, const PreM& v) {
+  const double dt = p.dt;
+  float2* A = (float2*)p.ynew;
+  if (STAGE == 0) {
+    A[g] = make_float2((float)(rx * (dt / 6)), (float)(ry * (dt / 6)));
+    return make_double2(v.y.x + rx * (dt / 2), v.y.y + ry * (dt / 2));
+  } else if (STAGE == 1) {
+    A[p.nstate + g] = make_float2((float)(rx * (dt / 3)), (float)(ry * (dt / 3)));
+    return make_double2(v.y.x + rx * (dt / 2), v.y.y + ry * (dt / 2));
+  } else if (STAGE == 2) {
+    A[p.nstate + g] = make_float2((float)((double)v.a.x + (double)v.b.x + rx * (dt / 3)),
+                                  (float)((double)v.a.y + (double)v.b.y + ry * (dt / 3)));
+    return make_double2(v.y.x + rx * dt, v.y.y + ry * dt);
+  } else {
+    const double2 yn = make_double2(v.y.x + ((double)v.b.x + rx * (dt / 6)), v.y.y + ((double)v.b.y + ry * (dt / 6)));
+    p.y[g] = yn;
+    return yn;
+  }
+}
+
+// Writes the RK4 register of state element g: value (x, y) as complex double, or as complex float
+// in DELTA mode.
+template <bool DELTA>
+__device__ __forceinline__ void st_reg(const StageParams& p, int64_t g, double x, double y) {
+  if constexpr (DELTA) {
+    ((float2*)p.ynew)[g] = make_float2((float)x, (float)y);
+  } else {
+    p.ynew[g] = make_double2(x, y);
+  }
 }
 
 // RK4 stage update of one element (i, j) from its right-hand side (rx, ry): gx is its index in the
@@ -336,22 +431,27 @@ __device__ __forceinline__ Pre rk4_prefetch(const StageParams& p, const StateRef
 // and its conjugate at (j, i), from the upper element only, so that this copy is exactly the one
 // the unpack kernel makes from y.  `save` marks ADO 0 of an instance on a trajectory step; tpos is
 // the element's trajectory slot and tposT that of (j, i).
-template <typename TO, int STAGE, bool PACK>
+template <typename TO, int STAGE, bool PACK, bool DELTA>
 __device__ __forceinline__ void rk4_store(const StageParams& p, int64_t gx, int64_t gxT, const StateRef& s, double rx,
-                                          double ry, const Pre& v, cpx_store<TO>* xout, bool save, int64_t tpos,
+                                          double ry, const Pre<DELTA>& v, cpx_store<TO>* xout, bool save, int64_t tpos,
                                           int64_t tposT) {
   const double dt = p.dt;
   if (STAGE == 4) {
     ((double2*)p.xout)[gx] = make_double2(rx, ry);
   } else if (STAGE == 0) {
-    if (!s.lower) p.ynew[s.g] = make_double2(v.y.x + rx * (dt / 6), v.y.y + ry * (dt / 6));
+    if (!s.lower) {
+      if (DELTA) st_reg<DELTA>(p, s.g, rx * (dt / 6), ry * (dt / 6));
+      else st_reg<DELTA>(p, s.g, v.y.x + rx * (dt / 6), v.y.y + ry * (dt / 6));
+    }
     st(xout + gx, mk<TO>((TO)(v.y.x + rx * (dt / 2)), (TO)(v.y.y + ry * (dt / 2))));
   } else if (STAGE == 1 || STAGE == 2) {
-    if (!s.lower) p.ynew[s.g] = make_double2(v.an.x + rx * (dt / 3), v.an.y + ry * (dt / 3));
+    if (!s.lower) st_reg<DELTA>(p, s.g, (double)v.an.x + rx * (dt / 3), (double)v.an.y + ry * (dt / 3));
     const double h = STAGE == 1 ? dt / 2 : dt;
     st(xout + gx, mk<TO>((TO)(v.y.x + rx * h), (TO)(v.y.y + ry * h)));
   } else {
-    const double2 yn = make_double2(v.an.x + rx * (dt / 6), v.an.y + ry * (dt / 6));
+    const double2 yn = DELTA ? make_double2(v.y.x + ((double)v.an.x + rx * (dt / 6)),
+                                            v.y.y + ((double)v.an.y + ry * (dt / 6)))
+                             : make_double2((double)v.an.x + rx * (dt / 6), (double)v.an.y + ry * (dt / 6));
     if (!s.lower) {
       p.y[s.g] = yn;
       // packed (Hermitian) mode: (j, i) is written as the conjugate of (i, j), except on the diagonal
@@ -371,18 +471,21 @@ __device__ __forceinline__ void rk4_store(const StageParams& p, int64_t gx, int6
 // (stage 3) and returns the element of the stage output (X' for stages 0-2, the new state for
 // stage 3, whose full-matrix copy is the next step's stage-0 input).  The lower element (j, i) of
 // the output is its conjugate.
-template <int STAGE>
-__device__ __forceinline__ double2 rk4_upper(const StageParams& p, int64_t g, double rx, double ry, const Pre& v) {
+template <int STAGE, bool DELTA>
+__device__ __forceinline__ double2 rk4_upper(const StageParams& p, int64_t g, double rx, double ry, const Pre<DELTA>& v) {
   const double dt = p.dt;
   if (STAGE == 0) {
-    p.ynew[g] = make_double2(v.y.x + rx * (dt / 6), v.y.y + ry * (dt / 6));
+    if (DELTA) st_reg<DELTA>(p, g, rx * (dt / 6), ry * (dt / 6));
+    else st_reg<DELTA>(p, g, v.y.x + rx * (dt / 6), v.y.y + ry * (dt / 6));
     return make_double2(v.y.x + rx * (dt / 2), v.y.y + ry * (dt / 2));
   } else if (STAGE == 1 || STAGE == 2) {
-    p.ynew[g] = make_double2(v.an.x + rx * (dt / 3), v.an.y + ry * (dt / 3));
+    st_reg<DELTA>(p, g, (double)v.an.x + rx * (dt / 3), (double)v.an.y + ry * (dt / 3));
     const double h = STAGE == 1 ? dt / 2 : dt;
     return make_double2(v.y.x + rx * h, v.y.y + ry * h);
   } else {
-    const double2 yn = make_double2(v.an.x + rx * (dt / 6), v.an.y + ry * (dt / 6));
+    const double2 yn = DELTA ? make_double2(v.y.x + ((double)v.an.x + rx * (dt / 6)),
+                                            v.y.y + ((double)v.an.y + ry * (dt / 6)))
+                             : make_double2((double)v.an.x + rx * (dt / 6), (double)v.an.y + ry * (dt / 6));
     p.y[g] = yn;
     return yn;
   }
@@ -394,8 +497,9 @@ __device__ __forceinline__ int packed_index(int i, int j, int d) { return ((i * 
 template <typename TO, int STAGE, bool PACK>
 __device__ __forceinline__ void rk4_apply(const StageParams& p, int64_t gx, int64_t gxT, const StateRef& s, double rx,
                                           double ry, cpx_store<TO>* xout, bool save, int64_t tpos, int64_t tposT) {
-  const Pre v = rk4_prefetch<STAGE>(p, s);
-  rk4_store<TO, STAGE, PACK>(p, gx, gxT, s, rx, ry, v, xout, save, tpos, tposT);
+  constexpr bool DELTA = std::is_same<TO, float>::value;
+  const Pre<DELTA> v = rk4_prefetch<STAGE, DELTA>(p, s);
+  rk4_store<TO, STAGE, PACK, DELTA>(p, gx, gxT, s, rx, ry, v, xout, save, tpos, tposT);
 }
 
 __device__ __forceinline__ void item_range(const StageParams& p, int& it0, int& itend, int& itstep) {
@@ -614,6 +718,8 @@ constexpr int proj_waves(bool mfma, bool f32, bool pack) {
 template <typename T, typename TI, typename TO, int STAGE, bool MFMA, bool HERM, bool PACK>
 __global__ __launch_bounds__(kThreads, proj_waves(MFMA, std::is_same<T, float>::value, PACK))
 void heom_proj_kernel(StageParams p) {
+  constexpr bool kDelta = std::is_same<TO, float>::value;  // mixed precision: FP32 RK4 increment
+  constexpr bool kPM = kDelta && PACK;  // packed mixed run: two FP32 registers (rk4_upper_m)
   extern __shared__ __align__(16) char smem[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: ADO indices stay scalar
@@ -735,11 +841,12 @@ void heom_proj_kernel(StageParams p) {
       };
       issue(0, 0);
       cpx<T> acc[4];
-      Pre pr[4];
+      Pre<kDelta> pr[4];
+      PreM pm[4];
       if (MFMA && !PACK && PQA_RK4_HOIST == 2) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (4 * gk + r < d && gi < d) pr[r] = rk4_prefetch<STAGE>(p, state_ref<PACK>(I, a, 4 * gk + r, gi));
+          if (4 * gk + r < d && gi < d) pr[r] = rk4_prefetch<STAGE, kDelta>(p, state_ref<PACK>(I, a, 4 * gk + r, gi));
       }
       if (a + kThreads / 64 < aend) prefetch(a + kThreads / 64);
       // ---- commutator and damping (while the neighbour loads are in flight)
@@ -786,18 +893,22 @@ void heom_proj_kernel(StageParams p) {
         }
       // RK4 operands of the lane's four elements, requested together (one round trip instead of
       // one per element); the gather below overlaps their latency
-      if (MFMA && PACK) {
+      if (MFMA && PACK && !(kPM && STAGE == 0)) {
         // packed mode: RK4 operands of this lane's upper elements, packed indices lane + 64q (d <= 16:
         // at most 136 of them), read in packed order
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
           const int pk = lane + 64 * q;
-          if (pk < ((d * (d + 1)) >> 1)) pr[q] = rk4_prefetch<STAGE>(p, StateRef{I.poff + (int64_t)a * I.dp + pk, false});
+          if (pk < ((d * (d + 1)) >> 1)) {
+            const int64_t g = I.poff + (int64_t)a * I.dp + pk;
+            if constexpr (kPM) pm[q] = rk4_prefetch_m<STAGE>(p, g);
+            else pr[q] = rk4_prefetch<STAGE, kDelta>(p, StateRef{g, false});
+          }
         }
       } else if (MFMA && PQA_RK4_HOIST == 1) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (4 * gk + r < d && gi < d) pr[r] = rk4_prefetch<STAGE>(p, state_ref<PACK>(I, a, 4 * gk + r, gi));
+          if (4 * gk + r < d && gi < d) pr[r] = rk4_prefetch<STAGE, kDelta>(p, state_ref<PACK>(I, a, 4 * gk + r, gi));
       }
       wave_sync();
       // ---- gather the neighbour terms from the row buffer, RK4 update
@@ -823,6 +934,19 @@ void heom_proj_kernel(StageParams p) {
           }
         };
         if (MFMA) {
+          // stages 0-2 of a packed mixed run: this lane's upper elements of the FP32 stage input (the
+          // own matrix), kept in FP32 and read before the tile takes its place
+          cpx<T> yo[3];
+          if constexpr (kPM && STAGE <= 2) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+              const int pk = lane + 64 * q;
+              if (pk < np) {
+                const int ij = sIJ[pk];
+                yo[q] = w.X[(ij >> 8) * rs + (ij & 255)];
+              }
+            }
+          }
           // commutator and damping of every element (MFMA lane layout) through the tile
           if (gi < d) {
 #pragma unroll
@@ -846,8 +970,14 @@ void heom_proj_kernel(StageParams p) {
             const int pk = lane + 64 * q;
             if (pk < np) {
               const int ij = sIJ[pk];
-              output(ij >> 8, ij & 255,
-                     rk4_upper<STAGE>(p, I.poff + (int64_t)a * I.dp + pk, (double)au[q].x, (double)au[q].y, pr[q]));
+              const int64_t g = I.poff + (int64_t)a * I.dp + pk;
+              if constexpr (kPM) {
+                PreM v = pm[q];
+                if (STAGE <= 2) v.y = y_from_input_m<STAGE>(yo[q], v);
+                output(ij >> 8, ij & 255, rk4_upper_m<STAGE>(p, g, (double)au[q].x, (double)au[q].y, v));
+              } else {
+                output(ij >> 8, ij & 255, rk4_upper<STAGE, kDelta>(p, g, (double)au[q].x, (double)au[q].y, pr[q]));
+              }
             }
           }
         } else {
@@ -855,7 +985,13 @@ void heom_proj_kernel(StageParams p) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
               const int pk = p0 + lane + 64 * q;
-              if (pk < np) pr[q] = rk4_prefetch<STAGE>(p, StateRef{I.poff + (int64_t)a * I.dp + pk, false});
+              if (pk < np) {
+                if constexpr (kPM) {
+                  pm[q] = rk4_prefetch_m<STAGE>(p, I.poff + (int64_t)a * I.dp + pk);
+                } else {
+                  pr[q] = rk4_prefetch<STAGE, kDelta>(p, StateRef{I.poff + (int64_t)a * I.dp + pk, false});
+                }
+              }
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -870,7 +1006,13 @@ void heom_proj_kernel(StageParams p) {
                 cpx<T> o = mul_mi(c);
                 cmsub(o, damp, w.X[i * rs + j]);
                 neighbours(o, i, j);
-                output(i, j, rk4_upper<STAGE>(p, I.poff + (int64_t)a * I.dp + pk, (double)o.x, (double)o.y, pr[q]));
+                const int64_t g = I.poff + (int64_t)a * I.dp + pk;
+                if constexpr (kPM) {
+                  if (STAGE <= 2) pm[q].y = y_from_input_m<STAGE>(w.X[i * rs + j], pm[q]);
+                  output(i, j, rk4_upper_m<STAGE>(p, g, (double)o.x, (double)o.y, pm[q]));
+                } else {
+                  output(i, j, rk4_upper<STAGE, kDelta>(p, g, (double)o.x, (double)o.y, pr[q]));
+                }
               }
             }
           }
@@ -906,8 +1048,8 @@ void heom_proj_kernel(StageParams p) {
             if (i < d) {
               const int e = i * d + gi;
               const StateRef sr = state_ref<PACK>(I, a, i, gi);
-              if (!PQA_RK4_HOIST) pr[r] = rk4_prefetch<STAGE>(p, sr);
-              rk4_store<TO, STAGE, PACK>(p, I.off + base + e, I.off + base + gi * d + i, sr, (double)acc[r].x,
+              if (!PQA_RK4_HOIST) pr[r] = rk4_prefetch<STAGE, kDelta>(p, sr);
+              rk4_store<TO, STAGE, PACK, kDelta>(p, I.off + base + e, I.off + base + gi * d + i, sr, (double)acc[r].x,
                                          (double)acc[r].y, pr[r], xout, save_ado, tbase + e, tbase + gi * d + i);
             }
           }
@@ -919,7 +1061,7 @@ void heom_proj_kernel(StageParams p) {
           for (int q = 0; q < 4; ++q) {
             const int e = e0 + lane + 64 * q;
             const int i = (int)(((uint32_t)e * dmag) >> 20);
-            if (e < d2) pr[q] = rk4_prefetch<STAGE>(p, state_ref<PACK>(I, a, i, e - i * d));
+            if (e < d2) pr[q] = rk4_prefetch<STAGE, kDelta>(p, state_ref<PACK>(I, a, i, e - i * d));
           }
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
@@ -936,7 +1078,7 @@ void heom_proj_kernel(StageParams p) {
               for (int t = w.gs[i]; t < w.gs[i + 1]; ++t) cmac(o, w.cl[t], w.R[t * rs + j]);
               for (int t = w.gs[j]; t < w.gs[j + 1]; ++t)
                 cmac(o, w.cr[t], HERM ? (i != j ? conjg(w.R[t * rs + i]) : w.R[t * rs + i]) : w.Cc[t * rs + i]);
-              rk4_store<TO, STAGE, PACK>(p, I.off + base + e, I.off + base + j * d + i, state_ref<PACK>(I, a, i, j),
+              rk4_store<TO, STAGE, PACK, kDelta>(p, I.off + base + e, I.off + base + j * d + i, state_ref<PACK>(I, a, i, j),
                                          (double)o.x, (double)o.y, pr[q], xout, save_ado, tbase + e, tbase + j * d + i);
             }
           }
@@ -1132,10 +1274,10 @@ __global__ __launch_bounds__(kThreads, PQA_PIPE_WAVES) void heom_pipe_kernel(Sta
       }
       // rows of ADO a arrive; then its RK4 operands are requested
       stash_rows(u, nval);
-      Pre pre[4];
+      Pre<true> pre[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        if (4 * gk + r < d && gi < d) pre[r] = rk4_prefetch<STAGE>(p, state_ref<false>(I, a, 4 * gk + r, gi));
+        if (4 * gk + r < d && gi < d) pre[r] = rk4_prefetch<STAGE, true>(p, state_ref<false>(I, a, 4 * gk + r, gi));
       wave_sync();
       // next ADO: sort its list, request its rows, fetch the list after it
       cpx<T> damp_n = mk<T>(0, 0);
@@ -1163,7 +1305,7 @@ __global__ __launch_bounds__(kThreads, PQA_PIPE_WAVES) void heom_pipe_kernel(Sta
           if (i < d) {
             for (int t = gsb[u][i]; t < gsb[u][i + 1]; ++t) cmac(acc[r], clb[u][t], Rb[u][t * rs + gi]);
             const int e = i * d + gi;
-            rk4_store<T, STAGE, false>(p, I.off + base + e, I.off + base + gi * d + i, state_ref<false>(I, a, i, gi),
+            rk4_store<T, STAGE, false, true>(p, I.off + base + e, I.off + base + gi * d + i, state_ref<false>(I, a, i, gi),
                                 (double)acc[r].x, (double)acc[r].y, pre[r], xout, save_ado, tbase + e, tbase + gi * d + i);
           }
         }
@@ -2120,6 +2262,7 @@ int run_typed(pyqed_heom_plan* P, double2* y, char* ws, const double* field, int
   p.dt = dt;
   p.y = y;
   p.ynew = ynew;
+  p.nstate = P->n_state;
   p.traj = traj;
   p.field = field;
   if (!P->resident.empty()) {
