@@ -219,18 +219,21 @@ def stream_probe(torch):
 
 
 LEGACY_METRIC = "ADO-updates/s, legacy single-exponential HEOM (oqs.HEOMSolver.run, forward Euler, FP64), 1 GPU"
+LEGACY_RK_METRIC = "ADO-updates/s, legacy single-exponential HEOM (HEOM.heom.HEOMSolver.run, RK4, FP64), 1 GPU"
 FP64_PEAK_TFLOPS = 78.6
 
 
-def legacy_workload(n_inst):
+def legacy_workload(n_inst, rk=False):
     """The spin-boson of pyqed's examples/heom.py:77-96 (H = -sx/2 - sz/2, S = sz, e_ops = [sz],
-    rho0 = |1><1|, dt = 0.02, cutoff 5, nado 5), swept over n_inst (temperature, reorganization) points."""
+    rho0 = |1><1|, dt = 0.02, cutoff 5, nado 5), swept over n_inst (temperature, reorganization) points.
+    rk: the RK4 copy (pyqed.HEOM.heom), whose D0 = reorg (2T - i gamma) takes T as k_B T in the
+    Hamiltonian's units, swept over kT = 0.5..2 instead of 300..900."""
     sx = np.array([[0, 1], [1, 0]], complex)
     sz = np.diag([1.0, -1.0]).astype(complex)
     H = -0.5 * sx - 0.5 * sz
     rho0 = np.diag([0.0, 1.0]).astype(complex)
     k = np.arange(n_inst)
-    temperature = 300.0 + 600.0 * (k % 64) / 63.0
+    temperature = (0.5 + 1.5 * (k % 64) / 63.0) if rk else (300.0 + 600.0 * (k % 64) / 63.0)
     reorg = 0.05 + 0.35 * (k // 64 % 64) / 63.0
     return dict(H=H, S=sz, e_ops=[sz], rho0=rho0, dt=0.02, cutoff=5.0, nado=5, temperature=temperature,
                 reorganization=reorg)
@@ -242,13 +245,17 @@ def legacy_flops(d, nado):
     return 8 * d ** 3 * (4 + 6 * (nado - 2))
 
 
-def legacy_cpu_baseline(w, budget_s=12.0):
+def legacy_cpu_baseline(w, budget_s=12.0, rk=False):
     from oracle import legacy_heom as LH
     n_done, t0, nt = 0, time.perf_counter(), 200
     while time.perf_counter() - t0 < budget_s:
         i = (n_done * 997) % len(w["temperature"])
-        gamma, D0 = LH.run_params(w["temperature"][i], w["cutoff"], w["reorganization"][i])
-        LH.heom_run(w["H"], w["rho0"], w["S"], w["e_ops"], gamma, D0, w["nado"], w["dt"], nt)
+        if rk:
+            gamma, D0 = LH.run_params_rk(w["temperature"][i], w["cutoff"], w["reorganization"][i])
+            LH.heom_run_rk4(w["H"], w["rho0"], w["S"], w["e_ops"], gamma, D0, w["nado"], w["dt"], nt)
+        else:
+            gamma, D0 = LH.run_params(w["temperature"][i], w["cutoff"], w["reorganization"][i])
+            LH.heom_run(w["H"], w["rho0"], w["S"], w["e_ops"], gamma, D0, w["nado"], w["dt"], nt)
         n_done += 1
     el = time.perf_counter() - t0
     return {"value": n_done * nt * (w["nado"] - 1) / el, "unit": "ADO-updates/s", "cores": 1, "kind": "port",
@@ -257,11 +264,13 @@ def legacy_cpu_baseline(w, budget_s=12.0):
 
 def bench_legacy(args, torch):
     from pyqed_amd import _lib
+    from pyqed_amd.HEOM.heom import rk_run_params
     from pyqed_amd.oqs import legacy_run_params
     L = _lib.lib()
-    w = legacy_workload(args.n_inst)
+    rk = args.workload == "legacy-rk4"
+    w = legacy_workload(args.n_inst, rk)
     n, d, nado = args.n_inst, w["H"].shape[0], w["nado"]
-    gamma, D0 = legacy_run_params(w["temperature"], w["cutoff"], w["reorganization"])
+    gamma, D0 = (rk_run_params if rk else legacy_run_params)(w["temperature"], w["cutoff"], w["reorganization"])
     params = np.stack([np.broadcast_to(gamma, (n,)), D0.real, D0.imag], axis=1)
     dev = torch.device("cuda", 0)
     A = torch.as_tensor(np.array(np.broadcast_to(w["H"], (n, d, d)), order="C"), device=dev)
@@ -275,7 +284,7 @@ def bench_legacy(args, torch):
     stream = torch.cuda.current_stream()
 
     def launch(nt):
-        _lib.check(L.pyqed_legacy_heom_run(0, d, nado, n, A.data_ptr(), S.data_ptr(), None, par.data_ptr(),
+        _lib.check(L.pyqed_legacy_heom_run(2 if rk else 0, d, nado, n, A.data_ptr(), S.data_ptr(), None, par.data_ptr(),
                                            ado.data_ptr(), nt, w["dt"], 1, E.data_ptr(), obs.data_ptr(),
                                            stream.cuda_stream))
 
@@ -289,23 +298,26 @@ def bench_legacy(args, torch):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     launch_ms = ev[0].elapsed_time(ev[1])
-    flops = legacy_flops(d, nado) * n * args.steps
+    flops = legacy_flops(d, nado) * n * args.steps * (4 if rk else 1)  # RK4: four right-hand sides per step
     achieved = flops / (launch_ms / 1e3) / 1e12
     out = {
-        "metric": LEGACY_METRIC, "value": n * (nado - 1) * args.steps / wall, "unit": "ADO-updates/s", "n_gpus": 1,
+        "metric": LEGACY_RK_METRIC if rk else LEGACY_METRIC, "value": n * (nado - 1) * args.steps / wall,
+        "unit": "ADO-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (pyqed examples/heom.py spin-boson over a temperature x reorganization grid)",
-        "config": {"workload": f"legacy spin-boson sweep: {n} instances, d={d}, nado={nado}, dt={w['dt']}, "
-                               f"one observable per step, {args.steps} steps in one launch", "instances": n},
-        "roofline": {"bound": "mfma", "kernel": f"legacy_heom_kernel<false, {'true' if d * d <= 256 else 'false'}>",
+        "config": {"workload": f"legacy spin-boson sweep{' (RK4 copy, kT 0.5..2)' if rk else ''}: {n} instances, "
+                               f"d={d}, nado={nado}, dt={w['dt']}, one observable per step, {args.steps} steps in "
+                               "one launch", "instances": n},
+        "roofline": {"bound": "mfma", "kernel": "legacy_heom_kernel<false, false, true>" if rk else
+                     f"legacy_heom_kernel<false, {'true' if d * d <= 256 else 'false'}, false>",
                      "launch_ms": launch_ms,
                      "flops_per_launch": flops, "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None},
     }
     if not args.no_cpu:
         print("[bench] cpu baseline leg", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = legacy_cpu_baseline(w)
+        out["cpu_baseline"] = legacy_cpu_baseline(w, rk=rk)
     print(json.dumps(out), flush=True)
 
 
@@ -435,7 +447,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; 2000 for --workload legacy)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c5", "legacy", "spectral", "krylov"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c5", "legacy", "legacy-rk4", "spectral",
+                                                              "krylov"])
     ap.add_argument("--n-inst", type=int, default=None, help="instances (default 4096; 65536 for legacy)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--state", default="auto", choices=["auto", "full"],
@@ -451,7 +464,7 @@ def main():
     ap.add_argument("--stream-probe", action="store_true",
                     help="only measure achievable HBM bandwidth: device copies and a 3-read/2-write stream")
     args = ap.parse_args()
-    legacy = args.workload == "legacy"
+    legacy = args.workload in ("legacy", "legacy-rk4")
     if args.steps is None:
         args.steps = 2000 if legacy else 20
     if args.n_inst is None:

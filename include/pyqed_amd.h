@@ -35,7 +35,7 @@ extern "C" {
 #define PYQED_EUNSUPPORTED 4 /* valid input that this build does not handle */
 
 #define PYQED_PREC_FP64 0  /* plain FP64: right-hand side, stage inputs and state all FP64 */
-#define PYQED_PREC_MIXED 1 /* FP32 right-hand side and stage inputs, FP64 state and RK4 accumulator */
+#define PYQED_PREC_MIXED 1 /* FP32 right-hand side, stage inputs and RK4 increments, FP64 state */
 
 /* Plan flags */
 #define PYQED_PLAN_NO_RESIDENT 1 /* never run instances LDS-resident (always stream through HBM) */
@@ -45,8 +45,8 @@ extern "C" {
                                     couplings, real expn and etar = conj(etal) every ADO then stays
                                     Hermitian, and the kernels read column terms as conjugated rows.
                                     Rejected (PYQED_EINVAL) when the coefficients do not allow it. */
-#define PYQED_PLAN_PACKED 16     /* with PYQED_PLAN_HERMITIAN: the RK4 state y (and ynew in the
-                                    workspace) holds only the upper triangle of every ADO, row by row,
+#define PYQED_PLAN_PACKED 16     /* with PYQED_PLAN_HERMITIAN: the RK4 state y (and the RK4 registers
+                                    in the workspace) hold only the upper triangle of every ADO, row by row,
                                     d(d+1)/2 complex elements per ADO; plan_query's n_elems and
                                     plan_layout's state offsets refer to that layout.  The stage inputs
                                     stay full matrices.  No counterpart in the reference (which stores
@@ -218,10 +218,11 @@ int pyqed_liouville_adjoint(int32_t N, int32_t nterm, const void* A, int32_t n_i
                             const void* rhoT_bar, double* grad_field, void* grad_A, void* rho0_bar,
                             void* ckpt, int64_t ckpt_bytes, void* stream);
 
-/* Legacy single-exponential HEOM of oqs.HEOMSolver, forward Euler with the
- * reference's in-place sweep over the hierarchy (ADO n sees ADO n-1 of the same
- * step and ADO n+1 of the previous one).  One workgroup per instance; the
- * whole hierarchy is held in LDS, so (nado + 4) * M^2 * 16 bytes must fit.
+/* Legacy single-exponential HEOM of oqs.HEOMSolver (modes 0, 1: forward Euler
+ * with the reference's in-place sweep over the hierarchy, ADO n seeing ADO n-1
+ * of the same step and ADO n+1 of the previous one) and of its second copy
+ * HEOM.heom.HEOMSolver (modes 2, 3).  One workgroup per instance; the whole
+ * hierarchy is held in LDS, so (nado + 4) * M^2 * 16 bytes must fit (modes 0, 1, 3).
  *   mode 0 replaces _heom (pyqed/oqs.py:1808-1874, called by HEOMSolver.run,
  *     oqs.py:1360-1364): M = d, A = H, S = c_ops[0], Sp unused,
  *     params[i] = (gamma, Re D0, Im D0), ado [n_inst][nado][d][d] starts with
@@ -231,6 +232,13 @@ int pyqed_liouville_adjoint(int32_t N, int32_t nterm, const void* A, int32_t n_i
  *     oqs.py:1366-1370): M = d^2, A = L0, S = Sm, Sp = the commutator and
  *     anticommutator superoperators of S, params[i] = (gamma, a, b), ado the
  *     propagator blocks u [n_inst][nado][M][M] (u[0] = identity); n_obs = 0.
+ *   mode 2 replaces _heom of the second copy, pyqed/HEOM/heom.py:275-346
+ *     (HEOM.heom.HEOMSolver.run, HEOM/heom.py:195-199): arguments as mode 0 with
+ *     D0 = reorg (2T - i gamma), classic RK4 over the whole hierarchy instead of
+ *     the sweep; (4 nado + 2) * M^2 * 16 bytes of LDS per instance.
+ *   mode 3 replaces _heom_propagator of that copy (HEOM/heom.py:349-413,
+ *     HEOMSolver.propagator, HEOM/heom.py:201-205): arguments as mode 1, u[0]
+ *     updated once per step.
  * A, S, Sp are device [n_inst][M][M] complex double; params is a device
  * [n_inst][3] double array; ado is advanced in place by nt steps of dt. */
 int pyqed_legacy_heom_run(int32_t mode, int32_t M, int32_t nado, int32_t n_inst, const void* A, const void* S,

@@ -429,3 +429,13 @@ class HEOMSolver:
         fin, _ = legacy_heom_launch(1, op2sop(H)[None], op2sop(S)[None], op2sop(S, "anticommutator")[None],
                                     [[gamma, a, b]], u, nt, dt)
         return fin[0]
+
+    def correlation_2op_1t(self, rho0, a_op, b_op, dt, Nt, output='cor.dat'):
+        """<A(t)B> of the Lindblad equation with H and c_ops (oqs.py:1381-1403 calls the Lindblad
+        regression routine of oqs.py:717-791): LindbladSolver.correlation_2op_1t."""
+        return LindbladSolver(self.H, self.c_ops).correlation_2op_1t(rho0, a_op, b_op, dt, Nt, output)
+
+    def correlation_3op_2t(self, rho0, ops, dt, Nt, Ntau):
+        """<A(t)B(t+tau)C(t)> of the Lindblad equation with H and c_ops (oqs.py:1405-1435, the same
+        regression as LindbladSolver.correlation_3op_2t)."""
+        return LindbladSolver(self.H, self.c_ops).correlation_3op_2t(rho0, ops, dt, Nt, Ntau)

@@ -118,3 +118,94 @@ def test_host_copies_are_row_major():
     a = host_c_order(np.broadcast_to(H, (3, 2, 2)))
     assert a.flags["C_CONTIGUOUS"] and a.strides == (64, 32, 16)
     assert not np.array(np.broadcast_to(H, (3, 2, 2)), np.complex128).flags["C_CONTIGUOUS"]
+
+
+def _taylor4(Z):
+    I = np.eye(Z.shape[0])
+    Z2 = Z @ Z
+    Z3 = Z2 @ Z
+    return I + Z + Z2 / 2 + Z3 / 6 + Z3 @ Z / 24
+
+
+def _dense_hierarchy_generator(H, S, gamma, D0, nado):
+    """dvec(X)/dt of HEOM/heom.py:_heom as one matrix over the stacked row-major vec(X_n), built from
+    Kronecker products: vec(A X) = (A x I) vec X, vec(X A) = (I x A^T) vec X."""
+    d = H.shape[0]
+    I = np.eye(d)
+    Lc = lambda A: np.kron(A, I) - np.kron(I, A.T)
+    La = lambda A: np.kron(A, I) + np.kron(I, A.T)
+    N = d * d
+    G = np.zeros((nado * N, nado * N), complex)
+    for n in range(nado - 1):
+        blk = lambda m: (slice(n * N, (n + 1) * N), slice(m * N, (m + 1) * N))
+        G[blk(n)] = -1j * Lc(H) - n * gamma * np.eye(N)
+        G[blk(n + 1)] = -Lc(S)
+        if n > 0:
+            G[blk(n - 1)] = n * (D0.real * Lc(S) + 1j * D0.imag * La(S))
+    return G
+
+
+def test_rk4_run_is_taylor_polynomial_power_of_hierarchy_generator():
+    rng = np.random.default_rng(5)
+    d, nado, dt, nt = 3, 5, 0.03, 40
+    H, S, rho0 = _rand_herm(d, rng), _rand_herm(d, rng), _rand_rho(d, rng)
+    gamma, D0 = LH.run_params_rk(0.8, 1.1, 0.3)
+    e = _rand_herm(d, rng)
+    obs, X = LH.heom_run_rk4(H, rho0, S, [e], gamma, D0, nado, dt, nt, return_ados=True)
+    G = _dense_hierarchy_generator(H, S, gamma, D0, nado)
+    y0 = np.zeros(nado * d * d, complex)
+    y0[:d * d] = rho0.reshape(-1)
+    ref = np.linalg.matrix_power(_taylor4(dt * G), nt) @ y0
+    assert np.abs(X.reshape(-1) - ref).max() < 1e-13
+    assert abs(obs[0, -1] - np.trace(e @ ref[:d * d].reshape(d, d))) < 1e-13
+    # the upper ADOs are populated and the last one stays at zero (F_{nado-1} = 0)
+    assert np.abs(X[1:nado - 1]).max() > 1e-3 and np.abs(X[-1]).max() == 0.0
+    # an Euler sweep on the same inputs is far from this
+    _, Xe = LH.heom_run(H, rho0, S, [], gamma, D0, nado, dt, nt, return_ados=True)
+    assert np.abs(Xe - X).max() > 1e-4
+
+
+def test_rk4_run_without_bath_and_params():
+    rng = np.random.default_rng(6)
+    d, dt, nt = 2, 0.05, 30
+    H, S, rho0 = _rand_herm(d, rng), _rand_herm(d, rng), _rand_rho(d, rng)
+    gamma, D0 = LH.run_params_rk(0.7, 0.9, 0.0)
+    _, X = LH.heom_run_rk4(H, rho0, S, [], gamma, D0, 4, dt, nt, return_ados=True)
+    L0 = np.kron(H, np.eye(d)) - np.kron(np.eye(d), H.T)
+    ref = np.linalg.matrix_power(_taylor4(-1j * dt * L0), nt) @ rho0.reshape(-1)
+    assert np.abs(X[0].reshape(-1) - ref).max() < 1e-13
+    # D0 = reorg (2 T - i gamma), temperature taken as k_B T
+    assert LH.run_params_rk(0.5, 1.0, 0.1) == (1.0, complex(0.1, -0.1))
+
+
+def test_rk4_run_pure_dephasing_fourth_order():
+    H = np.diag([0.0, 0.5, 1.2]).astype(complex)
+    S = np.diag([1.0, 0.0, -0.5]).astype(complex)
+    rho0 = _rand_rho(3, np.random.default_rng(2))
+    gamma, D0 = LH.run_params_rk(0.5, 1.0, 0.1)
+    T = 4.0
+    exact = LH.dephasing_exact(rho0, H, S, D0, gamma, T)
+    errs = []
+    for dt in (0.1, 0.05):
+        _, X = LH.heom_run_rk4(H, rho0, S, [], gamma, D0, 12, dt, int(round(T / dt)), return_ados=True)
+        errs.append(np.abs(X[0] - exact).max())
+    assert errs[0] < 1e-6 and errs[1] < 5e-8
+    assert 14.0 < errs[0] / errs[1] < 18.0                  # fourth order in dt
+    assert np.abs(exact - rho0 * np.exp(-1j * (np.diag(H)[:, None] - np.diag(H)[None, :]) * T)).max() > 0.05
+
+
+def test_propagator_heom_module_updates_u0_once():
+    rng = np.random.default_rng(7)
+    d, dt, nt = 2, 0.02, 25
+    H, S = _rand_herm(d, rng), _rand_herm(d, rng)
+    gamma, a, b = LH.propagator_params(300.0, 0.5, 0.0)
+    u = LH.heom_propagator(H, S, gamma, a, b, 4, dt, nt, n_first=1)
+    L0 = np.kron(H, np.eye(d)) - np.kron(np.eye(d), H.T)
+    step = np.eye(d * d) - 1j * dt * L0
+    assert np.abs(u[0] - np.linalg.matrix_power(step, nt)).max() < 1e-13
+    # with a bath, u[1] after one step is a x dt Sm (from the updated u[0]); the oqs copy's double
+    # update of u[0] changes u[1] at first order
+    gamma, a, b = LH.propagator_params(300.0, 0.5, 0.02)
+    u1 = LH.heom_propagator(H, S, gamma, a, b, 4, dt, 1, n_first=1)
+    Sm = np.kron(S, np.eye(d)) - np.kron(np.eye(d), S.T)
+    assert np.abs(u1[1] - a * dt * Sm @ step).max() < 1e-15
