@@ -592,6 +592,9 @@ constexpr int kMaxEnt = 64;  // neighbour entries per ADO handled by the project
 #ifndef PQA_PACK_WAVES
 #define PQA_PACK_WAVES 4  // the same for the packed-state MFMA form
 #endif
+#ifndef PQA_PACK_WAVES_LAST
+#define PQA_PACK_WAVES_LAST PQA_PACK_WAVES  // the same for its last RK4 stage
+#endif
 #ifndef PQA_CHUNK
 #define PQA_CHUNK 2
 #endif
@@ -711,12 +714,13 @@ __device__ __forceinline__ void proj_header(const StageParams& p, const InstDev&
 }
 
 // Minimum waves per SIMD each form of the projector kernel is compiled for.
-constexpr int proj_waves(bool mfma, bool f32, bool pack) {
-  return mfma ? (pack ? PQA_PACK_WAVES : PQA_PROJ_WAVES) : (f32 ? PQA_VALU_WAVES : PQA_PROJ_WAVES);
+constexpr int proj_waves(bool mfma, bool f32, bool pack, int stage) {
+  return mfma ? (pack ? (stage == 3 ? PQA_PACK_WAVES_LAST : PQA_PACK_WAVES) : PQA_PROJ_WAVES)
+              : (f32 ? PQA_VALU_WAVES : PQA_PROJ_WAVES);
 }
 
 template <typename T, typename TI, typename TO, int STAGE, bool MFMA, bool HERM, bool PACK>
-__global__ __launch_bounds__(kThreads, proj_waves(MFMA, std::is_same<T, float>::value, PACK))
+__global__ __launch_bounds__(kThreads, proj_waves(MFMA, std::is_same<T, float>::value, PACK, STAGE))
 void heom_proj_kernel(StageParams p) {
   constexpr bool kDelta = std::is_same<TO, float>::value;  // mixed precision: FP32 RK4 increment
   constexpr bool kPM = kDelta && PACK;  // packed mixed run: two FP32 registers (rk4_upper_m)
@@ -1571,6 +1575,10 @@ struct pyqed_heom_plan {
   int64_t cls_state[3] = {0, 0, 0};  // RK4 state elements of the streamed items per class
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // the generic class (small d, latency-bound launches) runs its stages on a stream of its own,
+  // concurrently with the projector classes: the classes share no elements
+  hipStream_t side0 = nullptr;
+  hipEvent_t ev_fork0 = nullptr, ev_join0 = nullptr;
   // per-launch timing (pyqed_heom_plan_set_timing): event pairs around every class launch
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
@@ -1639,6 +1647,9 @@ void destroy_plan(pyqed_heom_plan* P) {
   if (P->side) (void)hipStreamDestroy(P->side);
   if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
   if (P->ev_join) (void)hipEventDestroy(P->ev_join);
+  if (P->side0) (void)hipStreamDestroy(P->side0);
+  if (P->ev_fork0) (void)hipEventDestroy(P->ev_fork0);
+  if (P->ev_join0) (void)hipEventDestroy(P->ev_join0);
   for (hipEvent_t e : P->ev_pool) (void)hipEventDestroy(e);
   delete P;
 }
@@ -1939,7 +1950,10 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
   }
   if (hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&P->side0, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&P->ev_fork0, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&P->ev_join0, hipEventDisableTiming) != hipSuccess) {
     destroy_plan(P);
     return fail(PYQED_EHIP, "stream/event creation failed");
   }
@@ -2175,11 +2189,11 @@ struct LaunchTimer {
   }
 };
 
-// One RK4 stage over every streamed item class.  TI is the stage-input precision: FP64 for stage 0
-// of an unpacked plan (it reads y), the stage precision otherwise.
+// One RK4 stage over the streamed item classes c_lo <= c < c_hi.  TI is the stage-input precision:
+// FP64 for stage 0 of an unpacked plan (it reads y), the stage precision otherwise.
 template <typename T, int S, typename TI, bool PACK>
-int launch_stage_ti(pyqed_heom_plan* P, const StageParams& p0, hipStream_t st) {
-  for (int c = 0; c < 3; ++c) {
+int launch_stage_ti(pyqed_heom_plan* P, const StageParams& p0, hipStream_t st, int c_lo, int c_hi) {
+  for (int c = c_lo; c < c_hi; ++c) {
     const int n = P->cls_begin[c + 1] - P->cls_begin[c];
     if (n == 0) continue;
     StageParams p = p0;
@@ -2227,10 +2241,30 @@ int launch_stage_ti(pyqed_heom_plan* P, const StageParams& p0, hipStream_t st) {
 }
 
 template <typename T, int S>
-int launch_stage(pyqed_heom_plan* P, const StageParams& p0, hipStream_t st) {
-  if (P->packed) return launch_stage_ti<T, S, T, true>(P, p0, st);
+int launch_stage(pyqed_heom_plan* P, const StageParams& p0, hipStream_t st, int c_lo = 0, int c_hi = 3) {
+  if (P->packed) return launch_stage_ti<T, S, T, true>(P, p0, st, c_lo, c_hi);
   // unpacked stage 0 reads y itself (FP64)
-  return launch_stage_ti<T, S, typename std::conditional<S == 0, double, T>::type, false>(P, p0, st);
+  return launch_stage_ti<T, S, typename std::conditional<S == 0, double, T>::type, false>(P, p0, st, c_lo, c_hi);
+}
+
+// The stages of one RK4 step over the streamed classes c_lo <= c < c_hi, on stream st.
+template <typename T>
+int launch_step(pyqed_heom_plan* P, StageParams p, const void* x1, void* x0, bool save, int64_t slot,
+                hipStream_t st, int c_lo, int c_hi) {
+  int rc;
+  p.xin = P->packed ? x1 : (const void*)p.y;
+  p.xout = x0;
+  if ((rc = launch_stage<T, 0>(P, p, st, c_lo, c_hi))) return rc;
+  p.xin = x0;
+  p.xout = (void*)x1;
+  if ((rc = launch_stage<T, 1>(P, p, st, c_lo, c_hi))) return rc;
+  p.xin = x1;
+  p.xout = x0;
+  if ((rc = launch_stage<T, 2>(P, p, st, c_lo, c_hi))) return rc;
+  p.xin = x0;
+  p.xout = P->packed ? (void*)x1 : nullptr;
+  p.save_slot = save ? slot : -1;
+  return launch_stage<T, 3>(P, p, st, c_lo, c_hi);
 }
 
 // Packed mode: the stage-0 input of the first step, X = y as full matrices in the stage precision
@@ -2290,23 +2324,27 @@ int run_typed(pyqed_heom_plan* P, double2* y, char* ws, const double* field, int
       const int rc = launch(heom_unpack_kernel<T>, std::max(1, std::min(P->n_items_stream, P->num_cu * 8)), 0, st, u);
       if (rc) return rc;
     }
+    // the generic class on its own stream when other classes share the step
+    const bool split = P->cls_begin[1] > P->cls_begin[0] && P->cls_begin[3] > P->cls_begin[1];
+    if (split) {
+      PQA_HIP(hipEventRecord(P->ev_fork0, st));
+      PQA_HIP(hipStreamWaitEvent(P->side0, P->ev_fork0, 0));
+    }
     for (int64_t s = 0; s < nsteps; ++s) {
       p.step = step0 + s;
+      const bool save = traj && ((s + 1) % save_every) == 0;
+      const int64_t slot = save ? (s + 1) / save_every - 1 : -1;
       int rc;
-      p.xin = P->packed ? (const void*)x1 : (const void*)y;
-      p.xout = x0;
-      if ((rc = launch_stage<T, 0>(P, p, st))) return rc;
-      p.xin = x0;
-      p.xout = x1;
-      if ((rc = launch_stage<T, 1>(P, p, st))) return rc;
-      p.xin = x1;
-      p.xout = x0;
-      if ((rc = launch_stage<T, 2>(P, p, st))) return rc;
-      p.xin = x0;
-      p.xout = P->packed ? x1 : nullptr;
-      p.save_slot = (traj && ((s + 1) % save_every) == 0) ? (s + 1) / save_every - 1 : -1;
-      if ((rc = launch_stage<T, 3>(P, p, st))) return rc;
-      p.save_slot = -1;
+      if (split) {
+        if ((rc = launch_step<T>(P, p, x1, x0, save, slot, P->side0, 0, 1))) return rc;
+        if ((rc = launch_step<T>(P, p, x1, x0, save, slot, st, 1, 3))) return rc;
+      } else if ((rc = launch_step<T>(P, p, x1, x0, save, slot, st, 0, 3))) {
+        return rc;
+      }
+    }
+    if (split) {
+      PQA_HIP(hipEventRecord(P->ev_join0, P->side0));
+      PQA_HIP(hipStreamWaitEvent(st, P->ev_join0, 0));
     }
   }
   if (!P->resident.empty()) PQA_HIP(hipStreamWaitEvent(st, P->ev_join, 0));
