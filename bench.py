@@ -321,6 +321,104 @@ def bench_legacy(args, torch):
     print(json.dumps(out), flush=True)
 
 
+C4_METRIC = ("gradients/s, Lindblad adjoint (3-level Lambda system, 4000 RK4 steps, dJ/d 200 field amplitudes "
+             "+ 3 level energies, FP64), 1 GPU")
+
+
+def c4_cpu_baseline(w, prob, budget_s=10.0):
+    """The oracle's FP64 autograd (torch on the CPU, one thread) of the same objective, rho_22(T)."""
+    import torch
+    from oracle.lindblad_torch import final_state
+    nthr = torch.get_num_threads()
+    torch.set_num_threads(1)
+    rng = np.random.default_rng(45)
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        th = torch.tensor(w["eps"], dtype=torch.float64, requires_grad=True)
+        am = torch.tensor(rng.normal(0.0, 0.1, (1, w["nseg"])), dtype=torch.float64, requires_grad=True)
+        final_state(th, am, prob.base, prob.G, prob.B, w["rho0"], w["dt"], w["steps_per_seg"])[2, 2].real.backward()
+        n += 1
+    el = time.perf_counter() - t0
+    torch.set_num_threads(nthr)
+    return {"value": n / el, "unit": "gradients/s", "cores": 1, "kind": "port",
+            "sample": f"{n} gradients (forward + torch.autograd backward over the 4000 steps, FP64, one thread)"}
+
+
+def bench_c4(args, torch):
+    """Batched adjoint gradients of the C4 problem: n_inst seeded field-amplitude sets, one launch of
+    pyqed_liouville_adjoint per batch (forward sweep with checkpoints, then the backward sweep)."""
+    from pyqed_amd import _lib
+    from pyqed_amd import workloads as W
+    from pyqed_amd.adjoint import lambda_system
+    L = _lib.lib()
+    w = W.lambda_c4()
+    prob = lambda_system(w["eps"], w["mu"], list(w["c_ops"]), w["dt"], w["steps_per_seg"], w["nseg"],
+                         w["checkpoint_every"])
+    n, N, nterm, nsteps, ce = args.n_inst, prob.N, 2, prob.nsteps, prob.ckpt_every
+    rng = np.random.default_rng(44)
+    amps = rng.normal(0.0, 0.1, (n, w["nseg"]))
+    amps[0] = w["amps"]
+    dev = torch.device("cuda", 0)
+    A = torch.as_tensor(prob._gens(w["eps"]), device=dev)
+    f = torch.as_tensor(np.ascontiguousarray(np.repeat(amps, prob.sps, axis=1)[:, :, None]), device=dev)
+    rho = torch.as_tensor(np.tile(np.asarray(w["rho0"], np.complex128).reshape(1, -1), (n, 1)), device=dev)
+    bar_h = np.zeros((n, N), np.complex128)
+    bar_h[:, 2 * 3 + 2] = 1.0                      # J = Re rho_22(T)
+    bar = torch.as_tensor(bar_h, device=dev)
+    gf = torch.zeros((n, nsteps, nterm - 1), dtype=torch.float64, device=dev)
+    gA = torch.zeros((n, nterm, N, N), dtype=torch.complex128, device=dev)
+    r0b = torch.zeros((n, N), dtype=torch.complex128, device=dev)
+    nck = (nsteps + ce - 1) // ce + 1
+    ck_bytes = n * (nck + ce) * N * 16
+    ck = torch.empty(ck_bytes, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def launch():
+        _lib.check(L.pyqed_liouville_adjoint(N, nterm, A.data_ptr(), n, rho.data_ptr(), f.data_ptr(), nsteps,
+                                             w["dt"], ce, bar.data_ptr(), gf.data_ptr(), gA.data_ptr(),
+                                             r0b.data_ptr(), ck.data_ptr(), ck_bytes, stream.cuda_stream))
+
+    for _ in range(args.warmup):
+        launch()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for _ in range(args.steps):
+        launch()
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = ev[0].elapsed_time(ev[1]) / args.steps
+    # instance 0 against the single-instance path (LindbladAdjoint.gradient)
+    gt1, ga1, _ = prob.gradient(w["eps"], w["amps"][None], w["rho0"], bar_h[0].reshape(3, 3))
+    gA0 = gA[0, 0].cpu().numpy()
+    gt = np.array([np.real(np.sum(np.conj(gA0) * Gj)) for Gj in prob.G])
+    ga = gf[0, :, 0].cpu().numpy().reshape(w["nseg"], prob.sps).sum(axis=1)
+    check = float(max(np.abs(gt - gt1).max(), np.abs(ga - ga1[0]).max()))
+    # forward, checkpoint recompute, adjoint products, field-gradient products and generator-gradient
+    # outer products: five sweeps of 4 complex N x N products (8 N^2 flops each) per step
+    flops = 5 * 4 * 8 * N * N * nsteps * n
+    achieved = flops / (launch_ms / 1e3) / 1e12
+    out = {
+        "metric": C4_METRIC, "value": n * args.steps / wall, "unit": "gradients/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded field amplitudes N(0, 0.1) per instance)",
+        "config": {"workload": f"3-level Lambda system, Lindblad dephasing and decay, 200 segments x 20 steps "
+                               f"(dt 0.05), checkpoint every {ce}; {n} gradient problems per launch; a step "
+                               "is one batch", "instances": n},
+        "roofline": {"bound": "mfma", "kernel": "liouville_adjoint_kernel", "launch_ms": launch_ms,
+                     "flops_per_launch": flops, "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None},
+        "check_vs_single_instance": check,
+    }
+    if not args.no_cpu:
+        print("[bench] cpu baseline leg", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = c4_cpu_baseline(w, prob)
+    print(json.dumps(out), flush=True)
+
+
 SPECTRAL_METRIC = "Liouville-propagator entries/s, eigen-expansion U(t) = U1 e^{lambda t} U1^-1 (FP64), 1 GPU"
 
 
@@ -447,8 +545,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; 2000 for --workload legacy)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c5", "legacy", "legacy-rk4", "spectral",
-                                                              "krylov"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "legacy", "legacy-rk4",
+                                                              "spectral", "krylov"])
     ap.add_argument("--n-inst", type=int, default=None, help="instances (default 4096; 65536 for legacy)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--state", default="auto", choices=["auto", "full"],
@@ -466,7 +564,7 @@ def main():
     args = ap.parse_args()
     legacy = args.workload in ("legacy", "legacy-rk4")
     if args.steps is None:
-        args.steps = 2000 if legacy else 20
+        args.steps = 2000 if legacy else (3 if args.workload == "c4" else 20)
     if args.n_inst is None:
         args.n_inst = 65536 if legacy else 4096
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -493,6 +591,9 @@ def main():
         return
     if args.workload == "spectral":
         bench_spectral(args, torch)
+        return
+    if args.workload == "c4":
+        bench_c4(args, torch)
         return
     if args.workload == "krylov":
         bench_krylov(args, torch)
