@@ -125,6 +125,25 @@ __device__ __forceinline__ cpx<double> gen(const LvParams& p, const cpx<double>*
   return mk(a.x, a.y);
 }
 
+// sum_k A_i[r][k] x[k] (HT = false) or sum_k conj(A_i[k][r]) x[k] (HT = true), with four
+// independent partial sums: a single wave per SIMD is common here, and one accumulator would make
+// every complex multiply-add wait for the previous one
+template <bool ALDS, bool HT>
+__device__ __forceinline__ cpx<double> row_dot(const LvParams& p, const cpx<double>* sA, int i, int r,
+                                               const cpx<double>* x) {
+  const int N = p.N;
+  cpx<double> a0 = mk(0.0, 0.0), a1 = a0, a2 = a0, a3 = a0;
+  int k = 0;
+  for (; k + 3 < N; k += 4) {
+    cmac(a0, HT ? conjg(gen<ALDS>(p, sA, i, k, r)) : gen<ALDS>(p, sA, i, r, k), x[k]);
+    cmac(a1, HT ? conjg(gen<ALDS>(p, sA, i, k + 1, r)) : gen<ALDS>(p, sA, i, r, k + 1), x[k + 1]);
+    cmac(a2, HT ? conjg(gen<ALDS>(p, sA, i, k + 2, r)) : gen<ALDS>(p, sA, i, r, k + 2), x[k + 2]);
+    cmac(a3, HT ? conjg(gen<ALDS>(p, sA, i, k + 3, r)) : gen<ALDS>(p, sA, i, r, k + 3), x[k + 3]);
+  }
+  for (; k < N; ++k) cmac(a0, HT ? conjg(gen<ALDS>(p, sA, i, k, r)) : gen<ALDS>(p, sA, i, r, k), x[k]);
+  return (a0 + a1) + (a2 + a3);
+}
+
 // out = A(step) x   (row r by lane r of the group, r += gsz)
 template <bool ALDS>
 __device__ void matvec(const LvParams& p, const Grp& q, const cpx<double>* sA, const double* cf,
@@ -135,9 +154,7 @@ __device__ void matvec(const LvParams& p, const Grp& q, const cpx<double>* sA, c
     for (int i = 0; i < p.nterm; ++i) {
       const double c = coeff(cf, i);
       if (c == 0.0) continue;
-      cpx<double> si = mk(0.0, 0.0);
-      for (int k = 0; k < N; ++k) cmac(si, gen<ALDS>(p, sA, i, r, k), x[k]);
-      s = s + si * c;
+      s = s + row_dot<ALDS, false>(p, sA, i, r, x) * c;
     }
     out[r] = s;
   }
@@ -148,11 +165,7 @@ template <bool ALDS>
 __device__ void matvec_term(const LvParams& p, const Grp& q, const cpx<double>* sA, int i, const cpx<double>* x,
                             cpx<double>* out) {
   const int N = p.N;
-  for (int r = q.li; q.live && r < N; r += q.gsz) {
-    cpx<double> s = mk(0.0, 0.0);
-    for (int k = 0; k < N; ++k) cmac(s, gen<ALDS>(p, sA, i, r, k), x[k]);
-    out[r] = s;
-  }
+  for (int r = q.li; q.live && r < N; r += q.gsz) out[r] = row_dot<ALDS, false>(p, sA, i, r, x);
 }
 
 // out = A(step)^H v
@@ -165,16 +178,20 @@ __device__ void matvec_h(const LvParams& p, const Grp& q, const cpx<double>* sA,
     for (int i = 0; i < p.nterm; ++i) {
       const double f = coeff(cf, i);
       if (f == 0.0) continue;
-      cpx<double> si = mk(0.0, 0.0);
-      for (int r = 0; r < N; ++r) cmac(si, conjg(gen<ALDS>(p, sA, i, r, c)), v[r]);
-      s = s + si * f;
+      s = s + row_dot<ALDS, true>(p, sA, i, c, v) * f;
     }
     out[c] = s;
   }
 }
 
-// sum of v over the lanes of the group (every thread of the workgroup must call it)
+// sum of v over the lanes of the group (every thread of the workgroup must call it).  A group of at
+// most 64 lanes lies inside one wavefront (groups are aligned to gsz), so lane exchanges replace
+// the LDS tree and its barriers.
 __device__ double group_sum(const Grp& q, double v, double* red) {
+  if (q.gsz <= 64) {
+    for (int w = q.gsz / 2; w > 0; w >>= 1) v += __shfl_xor(v, w, q.gsz);
+    return v;
+  }
   red[q.tid] = v;
   __syncthreads();
   for (int w = q.gsz / 2; w > 0; w >>= 1) {
@@ -451,10 +468,10 @@ int lv_shape(int N, int* gsz) {
   return g < 64 ? 64 : g;
 }
 
-// rows per field window: about 8 KB of windows (n_win per group) per workgroup, 1..256 rows
+// rows per field window: about 2 KB of windows (n_win per group) per workgroup, 1..256 rows
 int field_window(int nterm, int groups, int n_win) {
   if (nterm <= 1) return 1;
-  const int rows = 1024 / (groups * n_win * (nterm - 1));
+  const int rows = 256 / (groups * n_win * (nterm - 1));
   return rows < 1 ? 1 : (rows > 256 ? 256 : rows);
 }
 
