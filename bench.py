@@ -610,7 +610,8 @@ def main():
           flush=True)
     eng.advance(args.warmup, dt, args.precision)
     torch.cuda.synchronize()
-    print("[bench] warmup done", file=sys.stderr, flush=True)
+    print(f"[bench] warmup done; state at {eng.y.data_ptr():#x}, workspace at {eng._ws.data_ptr():#x}",
+          file=sys.stderr, flush=True)
     if args.graph:
         eng.advance_graph(args.graph, dt, args.precision, chunk=args.graph)  # capture (+ one replay) untimed
         torch.cuda.synchronize()
