@@ -419,6 +419,89 @@ def bench_c4(args, torch):
     print(json.dumps(out), flush=True)
 
 
+LINDBLAD_METRIC = "instance-steps/s, driven Lindblad RK4 (3-level Lambda system, 3 observables per step, FP64), 1 GPU"
+
+
+def lindblad_cpu_baseline(w, budget_s=10.0):
+    """The oracle's driven-Lindblad restatement (numpy, one thread, dense d x d algebra per stage)."""
+    from oracle import lindblad as OL
+    rng = np.random.default_rng(46)
+    H0 = np.diag(w["eps"]).astype(np.complex128)
+    e_ops = [np.diag(np.eye(3)[j]).astype(np.complex128) for j in range(3)]
+    n, nt, t0 = 0, 400, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        amps = rng.normal(0.0, 0.1, w["nseg"])
+        f = lambda t, a=amps: a[min(int(t / w["dt"] - 0.5) // w["steps_per_seg"], w["nseg"] - 1)]
+        OL.run_driven([H0, [w["mu"], f]], w["rho0"], list(w["c_ops"]), w["dt"], nt, e_ops=e_ops)
+        n += 1
+    el = time.perf_counter() - t0
+    return {"value": n * nt / el, "unit": "instance-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} instances x {nt} steps of the same driven problem (numpy FP64 restatement, one thread)"}
+
+
+def bench_lindblad(args, torch):
+    """Driven Lindblad propagation (LindbladSolver.run with a field, oqs.py:1699-1806, in batch): the C4
+    system forward only, n_inst seeded field-amplitude sets, populations after every step."""
+    from pyqed_amd import _lib
+    from pyqed_amd import workloads as W
+    from pyqed_amd.adjoint import lambda_system
+    L = _lib.lib()
+    w = W.lambda_c4()
+    prob = lambda_system(w["eps"], w["mu"], list(w["c_ops"]), w["dt"], w["steps_per_seg"], w["nseg"],
+                         w["checkpoint_every"])
+    n, N, nterm, nsteps = args.n_inst, prob.N, 2, prob.nsteps
+    rng = np.random.default_rng(44)
+    amps = rng.normal(0.0, 0.1, (n, w["nseg"]))
+    dev = torch.device("cuda", 0)
+    A = torch.as_tensor(prob._gens(w["eps"]), device=dev)
+    f = torch.as_tensor(np.ascontiguousarray(np.repeat(amps, prob.sps, axis=1)[:, :, None]), device=dev)
+    rho0 = np.tile(np.asarray(w["rho0"], np.complex128).reshape(1, -1), (n, 1))
+    rho = torch.as_tensor(rho0, device=dev)
+    E = torch.as_tensor(np.array([np.diag(np.eye(3)[j]).astype(np.complex128).T.reshape(-1) for j in range(3)]),
+                        device=dev)
+    obs = torch.empty((n, nsteps + 1, 3), dtype=torch.complex128, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def launch():
+        rho.copy_(torch.as_tensor(rho0, device=dev))
+        _lib.check(L.pyqed_liouville_run(N, nterm, A.data_ptr(), n, rho.data_ptr(), f.data_ptr(), nsteps, w["dt"],
+                                         3, E.data_ptr(), obs.data_ptr(), None, 1, stream.cuda_stream))
+
+    for _ in range(args.warmup):
+        launch()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for _ in range(args.steps):
+        launch()
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = ev[0].elapsed_time(ev[1]) / args.steps
+    pops = obs[:, -1, :].real.sum(dim=1)
+    # one RK4 step: 4 products of the step's N x N generator (8 N^2 flops each)
+    flops = 4 * 8 * N * N * nsteps * n
+    achieved = flops / (launch_ms / 1e3) / 1e12
+    out = {
+        "metric": LINDBLAD_METRIC, "value": n * nsteps * args.steps / wall, "unit": "instance-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded field amplitudes N(0, 0.1) per instance)",
+        "config": {"workload": f"3-level Lambda system, Lindblad dephasing and decay, 200 field segments x 20 "
+                               f"steps (dt 0.05), populations every step; {n} instances per launch; a step is one "
+                               "launch of 4000 RK4 steps", "instances": n},
+        "roofline": {"bound": "mfma", "kernel": "liouville_forward_kernel", "launch_ms": launch_ms,
+                     "flops_per_launch": flops, "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None},
+        "trace_deviation": float((pops - 1.0).abs().max()),
+    }
+    if not args.no_cpu:
+        print("[bench] cpu baseline leg", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = lindblad_cpu_baseline(w)
+    print(json.dumps(out), flush=True)
+
+
 SPECTRAL_METRIC = "Liouville-propagator entries/s, eigen-expansion U(t) = U1 e^{lambda t} U1^-1 (FP64), 1 GPU"
 
 
@@ -545,8 +628,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; 2000 for --workload legacy)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "legacy", "legacy-rk4",
-                                                              "spectral", "krylov"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "lindblad", "legacy",
+                                                              "legacy-rk4", "spectral", "krylov"])
     ap.add_argument("--n-inst", type=int, default=None, help="instances (default 4096; 65536 for legacy)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--state", default="auto", choices=["auto", "full"],
@@ -564,7 +647,7 @@ def main():
     args = ap.parse_args()
     legacy = args.workload in ("legacy", "legacy-rk4")
     if args.steps is None:
-        args.steps = 2000 if legacy else (3 if args.workload == "c4" else 20)
+        args.steps = 2000 if legacy else (3 if args.workload in ("c4", "lindblad") else 20)
     if args.n_inst is None:
         args.n_inst = 65536 if legacy else 4096
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -594,6 +677,9 @@ def main():
         return
     if args.workload == "c4":
         bench_c4(args, torch)
+        return
+    if args.workload == "lindblad":
+        bench_lindblad(args, torch)
         return
     if args.workload == "krylov":
         bench_krylov(args, torch)
