@@ -547,7 +547,7 @@ def bench_spectral(args, torch):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded random eigenvectors, stable eigenvalues)",
         "config": {"workload": f"Redfield propagator expansion: N = {N} (16 levels), {ns} times, [a][b][t] layout"},
-        "roofline": {"bound": "mfma", "kernel": "spectral_contract_kernel (+ spectral_weights_kernel)",
+        "roofline": {"bound": "mfma", "kernel": "spectral_mfma_kernel (+ spectral_weights_kernel)",
                      "call_ms": call_ms, "flops_per_call": flops, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None},
     }

@@ -117,6 +117,90 @@ __global__ __launch_bounds__(kThreads) void spectral_contract_kernel(int ns, int
   }
 }
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// Wide outputs on FP64 MFMA (v_mfma_f64_16x16x4_f64: D[16][16] += A[16][4] B[4][16]; lane l holds
+// A[l % 16][l / 16], B[l / 16][l % 16] and D[l / 16 + 4 r][l % 16], r = 0..3 -- the output rows of
+// the FP64 form interleave by 4, unlike the FP32 16x16x4 form's D[4 (l / 16) + r][l % 16]).  A block of four
+// wavefronts computes a 64 (t) x 64 (b) tile of one row a, each wavefront a 32 x 32 quadrant as 2 x 2
+// MFMA tiles; the complex product takes four real MFMAs (re += Wr Cr - Wi Ci, im += Wr Ci + Wi Cr).
+// TFAST: the MFMA rows run along b and its columns along t (D = C^T W^T), so a lane's consecutive
+// neighbours store consecutive t; otherwise rows along t and columns along b.
+template <bool TFAST>
+__global__ __launch_bounds__(256) void spectral_mfma_kernel(int ns, int k, int ncol, const double2* W,
+                                                            const double2* A, const double2* B, double2* out,
+                                                            int64_t so_t, int64_t so_a, int64_t so_b) {
+  constexpr int TB = 64;                     // tile width in b
+  __shared__ cpx<double> sW[kTJ][kTT + 1];   // W[t0 + r][j0 + c] stored as [c][r]
+  __shared__ cpx<double> sC[kTJ][TB + 1];    // A[a][j0 + c] * B[j0 + c][b0 + q]
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int t0 = blockIdx.x * kTT, b0 = blockIdx.y * TB, a = blockIdx.z;
+  const int qt = 32 * (wv >> 1), qb = 32 * (wv & 1);  // the wavefront's quadrant
+  const int lr = lane & 15, lk = lane >> 4;
+  f64x4 re[2][2], im[2][2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      re[u][v] = f64x4{0.0, 0.0, 0.0, 0.0};
+      im[u][v] = f64x4{0.0, 0.0, 0.0, 0.0};
+    }
+  const double2* Arow = A + (int64_t)a * k;
+  for (int j0 = 0; j0 < k; j0 += kTJ) {
+    for (int e = tid; e < kTT * kTJ; e += 256) {
+      const int r = e / kTJ, c = e - (e / kTJ) * kTJ;
+      const int t = t0 + r, j = j0 + c;
+      double2 v = make_double2(0.0, 0.0);
+      if (t < ns && j < k) v = W[(int64_t)t * k + j];
+      sW[c][r] = mk(v.x, v.y);
+    }
+    for (int e = tid; e < kTJ * TB; e += 256) {
+      const int c = e / TB, q = e - (e / TB) * TB;
+      const int j = j0 + c, b = b0 + q;
+      cpx<double> v = mk(0.0, 0.0);
+      if (j < k && b < ncol) {
+        const double2 x = Arow[j], y = B[(int64_t)j * ncol + b];
+        v = mk(x.x, x.y) * mk(y.x, y.y);
+      }
+      sC[c][q] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kTJ / 4; ++ks) {
+      const int c = 4 * ks + lk;
+      cpx<double> w[2], x[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) w[u] = sW[c][qt + 16 * u + lr];
+#pragma unroll
+      for (int v = 0; v < 2; ++v) x[v] = sC[c][qb + 16 * v + lr];
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          // !TFAST: A operand W (rows t), B operand C (columns b); TFAST: A = C^T (rows b), B = W^T
+          const cpx<double> p = TFAST ? x[v] : w[u], q = TFAST ? w[u] : x[v];
+          re[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(p.x, q.x, re[u][v], 0, 0, 0);
+          re[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(-p.y, q.y, re[u][v], 0, 0, 0);
+          im[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(p.x, q.y, im[u][v], 0, 0, 0);
+          im[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(p.y, q.x, im[u][v], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = lk + 4 * r;  // MFMA output row of this lane's r-th value; column lr
+        const int t = t0 + qt + 16 * u + (TFAST ? lr : row);
+        const int b = b0 + qb + 16 * v + (TFAST ? row : lr);
+        if (t < ns && b < ncol)
+          out[(int64_t)t * so_t + (int64_t)a * so_a + (int64_t)b * so_b] = make_double2(re[u][v][r], im[u][v][r]);
+      }
+}
+
 }  // namespace
 
 extern "C" int64_t pyqed_spectral_work_bytes(int32_t k, int32_t ns) {
@@ -149,11 +233,11 @@ extern "C" int pyqed_spectral_sum(int32_t kind, int32_t nrow, int32_t k, int32_t
   const double2 *Wc = (const double2*)W, *Ac = (const double2*)A, *Bc = (const double2*)B;
   double2* O = (double2*)out;
   if (wide && tfast)
-    hipLaunchKernelGGL((spectral_contract_kernel<4, true>), grid, dim3(kThreads), 0, st, (int)ns, (int)k,
-                       (int)ncol, Wc, Ac, Bc, O, so_t, so_a, so_b);
+    hipLaunchKernelGGL((spectral_mfma_kernel<true>), grid, dim3(kThreads), 0, st, (int)ns, (int)k, (int)ncol, Wc, Ac,
+                       Bc, O, so_t, so_a, so_b);
   else if (wide)
-    hipLaunchKernelGGL((spectral_contract_kernel<4, false>), grid, dim3(kThreads), 0, st, (int)ns, (int)k,
-                       (int)ncol, Wc, Ac, Bc, O, so_t, so_a, so_b);
+    hipLaunchKernelGGL((spectral_mfma_kernel<false>), grid, dim3(kThreads), 0, st, (int)ns, (int)k, (int)ncol, Wc,
+                       Ac, Bc, O, so_t, so_a, so_b);
   else if (tfast)
     hipLaunchKernelGGL((spectral_contract_kernel<1, true>), grid, dim3(kThreads), 0, st, (int)ns, (int)k,
                        (int)ncol, Wc, Ac, Bc, O, so_t, so_a, so_b);
