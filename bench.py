@@ -309,8 +309,10 @@ def bench_legacy(args, torch):
         "config": {"workload": f"legacy spin-boson sweep{' (RK4 copy, kT 0.5..2)' if rk else ''}: {n} instances, "
                                f"d={d}, nado={nado}, dt={w['dt']}, one observable per step, {args.steps} steps in "
                                "one launch", "instances": n},
-        "roofline": {"bound": "mfma", "kernel": "legacy_heom_kernel<false, false, true>" if rk else
-                     f"legacy_heom_kernel<false, {'true' if d * d <= 256 else 'false'}, false>",
+        "roofline": {"bound": "mfma",
+                     "kernel": (f"legacy_heom_kernel<false, false, true, {2 if d == 2 and nado * d * d <= 256 else 0}>"
+                                if rk else f"legacy_heom_kernel<false, {'true' if d * d <= 256 else 'false'}, false, "
+                                           f"{2 if d == 2 else 0}>"),
                      "launch_ms": launch_ms,
                      "flops_per_launch": flops, "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None},

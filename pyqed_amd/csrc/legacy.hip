@@ -37,6 +37,9 @@ using namespace pqa;
 namespace {
 
 constexpr int kMaxThreads = 256;
+#ifndef PQA_LEGACY_MT
+#define PQA_LEGACY_MT 1  // 0: always the runtime-M kernels
+#endif
 
 struct LgParams {
   int M, nado, n_obs, n_inst;
@@ -63,11 +66,16 @@ __host__ __device__ constexpr int lds_blocks(int nado, bool prop, bool one, bool
 // ONE = true: every lane owns at most one element of its instance (M^2 <= gsz), the new value waits
 // in a register; otherwise lanes stride over the elements and new values go through LDS.
 // RK = true (with PROP = false): classic RK4 over the whole hierarchy instead of the Euler sweep.
+// MT > 0: M = MT at compile time, and each lane keeps the row and column of the operators its
+// element needs in registers (the small-d instances, where LDS reads and address arithmetic
+// dominate a step).  Needs one element per lane: ONE for the sweep, nado M^2 <= gsz for RK.
 // Instances are packed gsz lanes apart, so a group never straddles a wavefront when gsz <= 64.
-template <bool PROP, bool ONE, bool RK>
-__global__ __launch_bounds__(kMaxThreads) void legacy_heom_kernel(LgParams p) {
+template <bool PROP, bool ONE, bool RK, int MT>
+__global__ __launch_bounds__(kMaxThreads, MT > 0 ? 4 : 1) void legacy_heom_kernel(LgParams p) {
+  constexpr bool REG = MT > 0;
+  static_assert(!REG || ONE || RK, "register operators need one element per lane");
   extern __shared__ __align__(16) char smem[];
-  const int tid = threadIdx.x, nth = blockDim.x, M = p.M, M2 = M * M, nado = p.nado, gsz = p.gsz;
+  const int tid = threadIdx.x, nth = blockDim.x, M = REG ? MT : p.M, M2 = M * M, nado = p.nado, gsz = p.gsz;
   const int g = tid / gsz, li = tid - g * gsz;
   const int b = blockIdx.x * (nth / gsz) + g;
   const bool live = b < p.n_inst;  // lanes of a missing instance still take every barrier
@@ -100,6 +108,22 @@ __global__ __launch_bounds__(kMaxThreads) void legacy_heom_kernel(LgParams p) {
   }
   const double dt = p.dt;
   __syncthreads();
+  // REG: the lane's element (i, j) = (li / M, li % M) needs row i and column j of A and S (row i of P)
+  constexpr int NR = REG ? MT : 1;
+  cpx<double> rA[NR], cA[NR], rS[NR], cS[NR], rP[NR];
+  if constexpr (REG) {
+    const int e0 = RK ? li % (MT * MT) : li;  // RK: lane li holds pair (li / M^2, li % M^2)
+    const int i = e0 / MT, j = e0 - (e0 / MT) * MT;
+#pragma unroll
+    for (int k = 0; k < MT; ++k) {
+      const bool ok = live && li < (RK ? nado * M2 : M2);
+      rA[k] = ok ? sA[i * MT + k] : mk(0.0, 0.0);
+      cA[k] = ok ? sA[k * MT + j] : mk(0.0, 0.0);
+      rS[k] = ok ? sS[i * MT + k] : mk(0.0, 0.0);
+      cS[k] = ok ? sS[k * MT + j] : mk(0.0, 0.0);
+      rP[k] = ok && PROP ? sP[i * MT + k] : mk(0.0, 0.0);
+    }
+  }
 
   // right-hand side of element e of ADO n (n <= nado - 2) from the hierarchy src
   auto rhs_el = [&](const cpx<double>* src, int n, int e) {
@@ -108,7 +132,28 @@ __global__ __launch_bounds__(kMaxThreads) void legacy_heom_kernel(LgParams p) {
     const cpx<double>* Z = src + (size_t)(n > 0 ? n - 1 : 0) * M2;
     const int i = e / M, j = e - (e / M) * M;
     cpx<double> h = mk(0.0, 0.0), s = mk(0.0, 0.0), zc = mk(0.0, 0.0), za = mk(0.0, 0.0);
-    for (int k = 0; k < M; ++k) {
+    if constexpr (REG) {  // e is the lane's element: the operators' entries are its registers
+#pragma unroll
+      for (int k = 0; k < MT; ++k) {
+        cmac(h, rA[k], Xn[k * MT + j]);
+        cmac(s, rS[k], Y[k * MT + j]);
+        if (!PROP) {
+          cmsub(h, Xn[i * MT + k], cA[k]);
+          cmsub(s, Y[i * MT + k], cS[k]);
+        }
+        if (n > 0) {
+          if (!PROP) {
+            const cpx<double> l = rS[k] * Z[k * MT + j], r = Z[i * MT + k] * cS[k];
+            zc = zc + (l - r);
+            za = za + (l + r);
+          } else {
+            cmac(zc, rS[k], Z[k * MT + j]);
+            cmac(za, rP[k], Z[k * MT + j]);
+          }
+        }
+      }
+    }
+    for (int k = 0; !REG && k < M; ++k) {
       cmac(h, sA[i * M + k], Xn[k * M + j]);
       cmac(s, sS[i * M + k], Y[k * M + j]);
       if (!PROP) {
@@ -214,12 +259,12 @@ __global__ __launch_bounds__(kMaxThreads) void legacy_heom_kernel(LgParams p) {
     for (int e = li; e < nado * M2; e += gsz) gado[e] = make_double2(X[e].x, X[e].y);
 }
 
-template <bool PROP, bool ONE, bool RK>
+template <bool PROP, bool ONE, bool RK, int MT = 0>
 int launch_legacy(const LgParams& p, int blocks, int nth, size_t lds, hipStream_t stream) {
   if (lds > 65536)
-    PQA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&legacy_heom_kernel<PROP, ONE, RK>),
+    PQA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&legacy_heom_kernel<PROP, ONE, RK, MT>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL((legacy_heom_kernel<PROP, ONE, RK>), dim3(blocks), dim3(nth), lds, stream, p);
+  hipLaunchKernelGGL((legacy_heom_kernel<PROP, ONE, RK, MT>), dim3(blocks), dim3(nth), lds, stream, p);
   PQA_HIP(hipGetLastError());
   return PYQED_OK;
 }
@@ -280,10 +325,16 @@ extern "C" int pyqed_legacy_heom_run(int32_t mode, int32_t M, int32_t nado, int3
   p.obs_out = (double2*)obs_out;
   const int blocks = (n_inst + groups - 1) / groups;
   hipStream_t st = (hipStream_t)stream;
-  if (rk) return launch_legacy<false, false, true>(p, blocks, nth, lds, st);
+  if (rk) {
+    // one (ADO, element) pair per lane; larger M spill at the 4-wave register budget
+    if (PQA_LEGACY_MT && span <= gsz && M == 2) return launch_legacy<false, false, true, 2>(p, blocks, nth, lds, st);
+    return launch_legacy<false, false, true>(p, blocks, nth, lds, st);
+  }
+  if (!prop && PQA_LEGACY_MT && one && M == 2) return launch_legacy<false, true, false, 2>(p, blocks, nth, lds, st);
   if (!prop)
     return one ? launch_legacy<false, true, false>(p, blocks, nth, lds, st)
                : launch_legacy<false, false, false>(p, blocks, nth, lds, st);
+  if (PQA_LEGACY_MT && one && M == 4) return launch_legacy<true, true, false, 4>(p, blocks, nth, lds, st);
   return one ? launch_legacy<true, true, false>(p, blocks, nth, lds, st)
              : launch_legacy<true, false, false>(p, blocks, nth, lds, st);
 }
