@@ -24,8 +24,10 @@ Liouvillian applied to the regression state and the closed-form two-level correl
 <sigma+(t) sigma->  = exp((i w - G/2) t).  The reference's _lindblad cannot be
 used for golden vectors: with numpy 2 / scipy 1.15 its csr state makes np.dot(dense, csr) return
 object arrays and the coherent term vanishes; the dense semantics restated here are the intended
-ones.  redfield_tensor has no golden vector either (parity unpinned beyond the mathematical
-checks above)."""
+ones.  redfield_tensor has no golden vector either; besides trace preservation and the Gibbs
+steady state it is pinned by the closed-form two-level rates (H = diag(0, w), a = sigma_x: the excited
+population decays at 2 S(w), the ground one rises at 2 S(-w)), which fix the scale, the sign and
+the argument convention of the rates."""
 import numpy as np
 import scipy.linalg as sla
 

@@ -174,3 +174,22 @@ def test_correlation_3op_2t_against_expm():
         for m in range(Ntau):
             ref = np.trace(b @ OL.expm_evolve(L, cc @ rho_t @ a, m * dt))
             assert abs(got[j, m] - ref) < 1e-8 * max(1.0, abs(ref))  # RK4 error at dt = 0.02
+
+
+def test_redfield_two_level_rates_closed_form():
+    # H = diag(0, w), a = sigma_x, any spectrum S: with C[n, m] = S(E_m - E_n), l = C * a and
+    # D(rho) = -[a, l rho - rho l^dag] (oqs.py:519-570), the excited population decays as
+    # d rho_11/dt = -2 S(w) and the ground population fills as d rho_00/dt = +2 S(w); from the ground
+    # state the up rate is 2 S(-w).  These fix the scale, the sign and the argument of the rates, which
+    # the trace and Gibbs checks leave open.
+    w0 = 0.8
+    H = np.diag([0.0, w0]).astype(complex)
+    sx = np.array([[0, 1], [1, 0]], complex)
+    S = lambda w: 0.03 + 0.01 * w + 0.002 * w * w    # no symmetry: S(w) != S(-w)
+    R, ev = OL.redfield_tensor(H, [sx], [S])
+    assert np.allclose(np.abs(ev), np.eye(2))        # eigenbasis = computational basis up to phases
+    v = lambda r: np.asarray(r, complex).reshape(-1)
+    d1 = (R @ v(np.diag([0.0, 1.0]))).reshape(2, 2)
+    assert abs(d1[1, 1] - (-2 * S(w0))) < 1e-14 and abs(d1[0, 0] - 2 * S(w0)) < 1e-14
+    d0 = (R @ v(np.diag([1.0, 0.0]))).reshape(2, 2)
+    assert abs(d0[0, 0] - (-2 * S(-w0))) < 1e-14 and abs(d0[1, 1] - 2 * S(-w0)) < 1e-14
