@@ -117,9 +117,10 @@ __device__ void gf_move(const LvParams& p, const Grp& q, Win& g, int64_t s) {
 __device__ __forceinline__ double coeff(const double* cf, int i) { return i == 0 ? 1.0 : cf[i - 1]; }
 
 // generator element A_i[r][c]: from the workgroup's LDS copy (ALDS) or global memory
-template <bool ALDS>
+template <bool ALDS, int NT>
 __device__ __forceinline__ cpx<double> gen(const LvParams& p, const cpx<double>* sA, int i, int r, int c) {
-  const int64_t e = ((int64_t)i * p.N + r) * p.N + c;
+  const int N = NT > 0 ? NT : p.N;
+  const int64_t e = ((int64_t)i * N + r) * N + c;
   if (ALDS) return sA[e];
   const double2 a = p.A[e];
   return mk(a.x, a.y);
@@ -128,57 +129,57 @@ __device__ __forceinline__ cpx<double> gen(const LvParams& p, const cpx<double>*
 // sum_k A_i[r][k] x[k] (HT = false) or sum_k conj(A_i[k][r]) x[k] (HT = true), with four
 // independent partial sums: a single wave per SIMD is common here, and one accumulator would make
 // every complex multiply-add wait for the previous one
-template <bool ALDS, bool HT>
+template <bool ALDS, bool HT, int NT>
 __device__ __forceinline__ cpx<double> row_dot(const LvParams& p, const cpx<double>* sA, int i, int r,
                                                const cpx<double>* x) {
-  const int N = p.N;
+  const int N = NT > 0 ? NT : p.N;
   cpx<double> a0 = mk(0.0, 0.0), a1 = a0, a2 = a0, a3 = a0;
   int k = 0;
   for (; k + 3 < N; k += 4) {
-    cmac(a0, HT ? conjg(gen<ALDS>(p, sA, i, k, r)) : gen<ALDS>(p, sA, i, r, k), x[k]);
-    cmac(a1, HT ? conjg(gen<ALDS>(p, sA, i, k + 1, r)) : gen<ALDS>(p, sA, i, r, k + 1), x[k + 1]);
-    cmac(a2, HT ? conjg(gen<ALDS>(p, sA, i, k + 2, r)) : gen<ALDS>(p, sA, i, r, k + 2), x[k + 2]);
-    cmac(a3, HT ? conjg(gen<ALDS>(p, sA, i, k + 3, r)) : gen<ALDS>(p, sA, i, r, k + 3), x[k + 3]);
+    cmac(a0, HT ? conjg(gen<ALDS, NT>(p, sA, i, k, r)) : gen<ALDS, NT>(p, sA, i, r, k), x[k]);
+    cmac(a1, HT ? conjg(gen<ALDS, NT>(p, sA, i, k + 1, r)) : gen<ALDS, NT>(p, sA, i, r, k + 1), x[k + 1]);
+    cmac(a2, HT ? conjg(gen<ALDS, NT>(p, sA, i, k + 2, r)) : gen<ALDS, NT>(p, sA, i, r, k + 2), x[k + 2]);
+    cmac(a3, HT ? conjg(gen<ALDS, NT>(p, sA, i, k + 3, r)) : gen<ALDS, NT>(p, sA, i, r, k + 3), x[k + 3]);
   }
-  for (; k < N; ++k) cmac(a0, HT ? conjg(gen<ALDS>(p, sA, i, k, r)) : gen<ALDS>(p, sA, i, r, k), x[k]);
+  for (; k < N; ++k) cmac(a0, HT ? conjg(gen<ALDS, NT>(p, sA, i, k, r)) : gen<ALDS, NT>(p, sA, i, r, k), x[k]);
   return (a0 + a1) + (a2 + a3);
 }
 
 // out = A(step) x   (row r by lane r of the group, r += gsz)
-template <bool ALDS>
+template <bool ALDS, int NT>
 __device__ void matvec(const LvParams& p, const Grp& q, const cpx<double>* sA, const double* cf,
                        const cpx<double>* x, cpx<double>* out) {
-  const int N = p.N;
+  const int N = NT > 0 ? NT : p.N;
   for (int r = q.li; q.live && r < N; r += q.gsz) {
     cpx<double> s = mk(0.0, 0.0);
     for (int i = 0; i < p.nterm; ++i) {
       const double c = coeff(cf, i);
       if (c == 0.0) continue;
-      s = s + row_dot<ALDS, false>(p, sA, i, r, x) * c;
+      s = s + row_dot<ALDS, false, NT>(p, sA, i, r, x) * c;
     }
     out[r] = s;
   }
 }
 
 // out = A_i x for one generator
-template <bool ALDS>
+template <bool ALDS, int NT>
 __device__ void matvec_term(const LvParams& p, const Grp& q, const cpx<double>* sA, int i, const cpx<double>* x,
                             cpx<double>* out) {
-  const int N = p.N;
-  for (int r = q.li; q.live && r < N; r += q.gsz) out[r] = row_dot<ALDS, false>(p, sA, i, r, x);
+  const int N = NT > 0 ? NT : p.N;
+  for (int r = q.li; q.live && r < N; r += q.gsz) out[r] = row_dot<ALDS, false, NT>(p, sA, i, r, x);
 }
 
 // out = A(step)^H v
-template <bool ALDS>
+template <bool ALDS, int NT>
 __device__ void matvec_h(const LvParams& p, const Grp& q, const cpx<double>* sA, const double* cf,
                          const cpx<double>* v, cpx<double>* out) {
-  const int N = p.N;
+  const int N = NT > 0 ? NT : p.N;
   for (int c = q.li; q.live && c < N; c += q.gsz) {
     cpx<double> s = mk(0.0, 0.0);
     for (int i = 0; i < p.nterm; ++i) {
       const double f = coeff(cf, i);
       if (f == 0.0) continue;
-      s = s + row_dot<ALDS, true>(p, sA, i, c, v) * f;
+      s = s + row_dot<ALDS, true, NT>(p, sA, i, c, v) * f;
     }
     out[c] = s;
   }
@@ -203,9 +204,9 @@ __device__ double group_sum(const Grp& q, double v, double* red) {
   return r;
 }
 
-template <bool ALDS>
+template <bool ALDS, int NT>
 __device__ void observe(const LvParams& p, const Grp& q, const cpx<double>* v, int64_t row, double* red) {
-  const int N = p.N;
+  const int N = NT > 0 ? NT : p.N;
   for (int o = 0; o < p.n_obs; ++o) {
     cpx<double> s = mk(0.0, 0.0);
     for (int e = q.li; q.live && e < N; e += q.gsz) {
@@ -219,33 +220,33 @@ __device__ void observe(const LvParams& p, const Grp& q, const cpx<double>* v, i
 }
 
 // One RK4 step in place on v (LDS), field row cf.  Scratch: X, k, acc (each N).
-template <bool ALDS>
+template <bool ALDS, int NT>
 __device__ void rk4_step(const LvParams& p, const Grp& q, const cpx<double>* sA, const double* cf, cpx<double>* v,
                          cpx<double>* X, cpx<double>* k, cpx<double>* acc) {
-  const int N = p.N;
+  const int N = NT > 0 ? NT : p.N;
   const double dt = p.dt, dt2 = p.dt / 2.0;
-  matvec<ALDS>(p, q, sA, cf, v, k);  // k1
+  matvec<ALDS, NT>(p, q, sA, cf, v, k);  // k1
   __syncthreads();
   for (int e = q.li; q.live && e < N; e += q.gsz) {
     acc[e] = k[e];
     X[e] = v[e] + k[e] * dt2;
   }
   __syncthreads();
-  matvec<ALDS>(p, q, sA, cf, X, k);  // k2
+  matvec<ALDS, NT>(p, q, sA, cf, X, k);  // k2
   __syncthreads();
   for (int e = q.li; q.live && e < N; e += q.gsz) {
     acc[e] = acc[e] + k[e] * 2.0;
     X[e] = v[e] + k[e] * dt2;
   }
   __syncthreads();
-  matvec<ALDS>(p, q, sA, cf, X, k);  // k3
+  matvec<ALDS, NT>(p, q, sA, cf, X, k);  // k3
   __syncthreads();
   for (int e = q.li; q.live && e < N; e += q.gsz) {
     acc[e] = acc[e] + k[e] * 2.0;
     X[e] = v[e] + k[e] * dt;
   }
   __syncthreads();
-  matvec<ALDS>(p, q, sA, cf, X, k);  // k4
+  matvec<ALDS, NT>(p, q, sA, cf, X, k);  // k4
   __syncthreads();
   for (int e = q.li; q.live && e < N; e += q.gsz) {
     const cpx<double> a = acc[e] + k[e];
@@ -255,19 +256,20 @@ __device__ void rk4_step(const LvParams& p, const Grp& q, const cpx<double>* sA,
 }
 
 // LDS copy of the generators (shared by every instance of the workgroup)
-template <bool ALDS>
+template <bool ALDS, int NT>
 __device__ void stage_generators(const LvParams& p, cpx<double>* sA) {
   if (!ALDS) return;
-  const int64_t n = (int64_t)p.nterm * p.N * p.N;
+  const int N = NT > 0 ? NT : p.N;
+  const int64_t n = (int64_t)p.nterm * N * N;
   for (int64_t e = threadIdx.x; e < n; e += blockDim.x) sA[e] = mk(p.A[e].x, p.A[e].y);
 }
 
 // LDS layout: [generators (ALDS)] [per group: 4 N state] [red: blockDim doubles]
 // [per group: field window, ch (nterm - 1) doubles]
-template <bool ALDS>
+template <bool ALDS, int NT>
 __global__ __launch_bounds__(kT) void liouville_forward_kernel(LvParams p) {
   extern __shared__ __align__(16) char smem[];
-  const int N = p.N;
+  const int N = NT > 0 ? NT : p.N;
   const Grp q = group_of(p);
   const int groups = blockDim.x / p.gsz;
   const int g = q.tid / p.gsz, nt1 = p.nterm - 1;
@@ -279,15 +281,15 @@ __global__ __launch_bounds__(kT) void liouville_forward_kernel(LvParams p) {
   cpx<double>* acc = k + N;
   double* red = (double*)(base + (int64_t)groups * 4 * N);
   Win cw{red + blockDim.x + (int64_t)g * p.ch * nt1, 0, 0};
-  stage_generators<ALDS>(p, sA);
+  stage_generators<ALDS, NT>(p, sA);
   double2* rho = p.rho + q.b * N;
   for (int e = q.li; q.live && e < N; e += q.gsz) v[e] = mk(rho[e].x, rho[e].y);
   __syncthreads();
-  if (p.n_obs) observe<ALDS>(p, q, v, 0, red);
+  if (p.n_obs) observe<ALDS, NT>(p, q, v, 0, red);
   for (int64_t s = 0; s < p.nsteps; ++s) {
     if (nt1) cf_ensure(p, q, cw, s, false);
-    rk4_step<ALDS>(p, q, sA, nt1 ? cw.w + (s - cw.lo) * nt1 : nullptr, v, X, k, acc);
-    if (p.n_obs) observe<ALDS>(p, q, v, s + 1, red);
+    rk4_step<ALDS, NT>(p, q, sA, nt1 ? cw.w + (s - cw.lo) * nt1 : nullptr, v, X, k, acc);
+    if (p.n_obs) observe<ALDS, NT>(p, q, v, s + 1, red);
     if (p.rho_out && ((s + 1) % p.save_every) == 0) {
       const int64_t slot = (s + 1) / p.save_every - 1;
       const int64_t nslot = p.nsteps / p.save_every;
@@ -307,10 +309,10 @@ __device__ double redot(const Grp& q, const cpx<double>* a, const cpx<double>* b
 
 // LDS layout: [generators (ALDS)] [per group: 11 N state, then nterm N^2 gradient accumulators (GALDS)]
 // [red: blockDim doubles] [per group: field window and field-gradient window, ch (nterm - 1) doubles each]
-template <bool ALDS, bool GALDS>
+template <bool ALDS, bool GALDS, int NT>
 __global__ __launch_bounds__(kT) void liouville_adjoint_kernel(LvParams p) {
   extern __shared__ __align__(16) char smem[];
-  const int N = p.N;
+  const int N = NT > 0 ? NT : p.N;
   const Grp q = group_of(p);
   const int groups = blockDim.x / p.gsz;
   const int64_t NN = (int64_t)N * N;
@@ -333,7 +335,7 @@ __global__ __launch_bounds__(kT) void liouville_adjoint_kernel(LvParams p) {
   const int g = q.tid / p.gsz, nt1 = p.nterm - 1;
   Win cw{red + blockDim.x + (int64_t)g * 2 * p.ch * nt1, 0, 0};
   Win gw{cw.w + (int64_t)p.ch * nt1, p.nsteps, p.nsteps};
-  stage_generators<ALDS>(p, sA);
+  stage_generators<ALDS, NT>(p, sA);
   const int64_t b = q.b;
   const double dt = p.dt;
   const int ce = p.ckpt_every;
@@ -348,7 +350,7 @@ __global__ __launch_bounds__(kT) void liouville_adjoint_kernel(LvParams p) {
   __syncthreads();
   for (int64_t s = 0; s < p.nsteps; ++s) {
     if (nt1) cf_ensure(p, q, cw, s, false);
-    rk4_step<ALDS>(p, q, sA, nt1 ? cw.w + (s - cw.lo) * nt1 : nullptr, v, X1, k, acc);
+    rk4_step<ALDS, NT>(p, q, sA, nt1 ? cw.w + (s - cw.lo) * nt1 : nullptr, v, X1, k, acc);
     if ((s + 1) % ce == 0 && (s + 1) / ce < p.nck)
       for (int e = q.li; q.live && e < N; e += q.gsz) ck[((s + 1) / ce) * N + e] = make_double2(v[e].x, v[e].y);
   }
@@ -379,7 +381,7 @@ __global__ __launch_bounds__(kT) void liouville_adjoint_kernel(LvParams p) {
       __syncthreads();
       if (s + 1 < s1) {
         if (nt1) cf_ensure(p, q, cw, s, false);
-        rk4_step<ALDS>(p, q, sA, nt1 ? cw.w + (s - cw.lo) * nt1 : nullptr, v, X1, k, acc);
+        rk4_step<ALDS, NT>(p, q, sA, nt1 ? cw.w + (s - cw.lo) * nt1 : nullptr, v, X1, k, acc);
       }
     }
     __syncthreads();
@@ -396,15 +398,15 @@ __global__ __launch_bounds__(kT) void liouville_adjoint_kernel(LvParams p) {
         X1[e] = mk(r.x, r.y);
       }
       __syncthreads();
-      matvec<ALDS>(p, q, sA, cf, X1, k);
+      matvec<ALDS, NT>(p, q, sA, cf, X1, k);
       __syncthreads();
       for (int e = q.li; q.live && e < N; e += q.gsz) X2[e] = X1[e] + k[e] * (dt / 2.0);
       __syncthreads();
-      matvec<ALDS>(p, q, sA, cf, X2, k);
+      matvec<ALDS, NT>(p, q, sA, cf, X2, k);
       __syncthreads();
       for (int e = q.li; q.live && e < N; e += q.gsz) X3[e] = X1[e] + k[e] * (dt / 2.0);
       __syncthreads();
-      matvec<ALDS>(p, q, sA, cf, X3, k);
+      matvec<ALDS, NT>(p, q, sA, cf, X3, k);
       __syncthreads();
       for (int e = q.li; q.live && e < N; e += q.gsz) {
         X4[e] = X1[e] + k[e] * dt;
@@ -419,7 +421,7 @@ __global__ __launch_bounds__(kT) void liouville_adjoint_kernel(LvParams p) {
         cpx<double>* Xst = Xs[st];
         // field gradients: Re <kbar_s, A_i X_s>
         for (int i = 1; i < p.nterm; ++i) {
-          matvec_term<ALDS>(p, q, sA, i, Xst, k);
+          matvec_term<ALDS, NT>(p, q, sA, i, Xst, k);
           __syncthreads();
           const double gi = redot(q, kb, k, N, red);
           if (q.live && q.li == 0) gw.w[(s - gw.lo) * nt1 + (i - 1)] += gi;
@@ -440,7 +442,7 @@ __global__ __launch_bounds__(kT) void liouville_adjoint_kernel(LvParams p) {
             }
           }
         }
-        matvec_h<ALDS>(p, q, sA, cf, kb, Xb);
+        matvec_h<ALDS, NT>(p, q, sA, cf, kb, Xb);
         __syncthreads();
         for (int e = q.li; q.live && e < N; e += q.gsz) {
           ub[e] = ub[e] + Xb[e];
@@ -531,8 +533,11 @@ extern "C" int pyqed_liouville_run(int32_t N, int32_t nterm, const void* A, int3
   const size_t lds = (alds ? a_bytes : 0) + st_bytes;
   if (lds > maxl) return fail(PYQED_EINVAL, "liouville_run: the state does not fit one workgroup's LDS");
   hipStream_t st = (hipStream_t)stream;
-  return alds ? launch_lv(liouville_forward_kernel<true>, blocks, nth, lds, st, p)
-              : launch_lv(liouville_forward_kernel<false>, blocks, nth, lds, st, p);
+  // N = 4 (a qubit) and N = 9 (three levels) at compile time: the row products unroll fully
+  if (alds && N == 4) return launch_lv(liouville_forward_kernel<true, 4>, blocks, nth, lds, st, p);
+  if (alds && N == 9) return launch_lv(liouville_forward_kernel<true, 9>, blocks, nth, lds, st, p);
+  return alds ? launch_lv(liouville_forward_kernel<true, 0>, blocks, nth, lds, st, p)
+              : launch_lv(liouville_forward_kernel<false, 0>, blocks, nth, lds, st, p);
 }
 
 extern "C" int pyqed_liouville_adjoint(int32_t N, int32_t nterm, const void* A, int32_t n_inst, const void* rho0,
@@ -585,8 +590,10 @@ extern "C" int pyqed_liouville_adjoint(int32_t N, int32_t nterm, const void* A, 
   const size_t lds = (alds ? a_bytes : 0) + st2;
   if (lds > maxl) return fail(PYQED_EINVAL, "liouville_adjoint: the state does not fit one workgroup's LDS");
   hipStream_t st = (hipStream_t)stream;
-  if (alds && galds) return launch_lv(liouville_adjoint_kernel<true, true>, blocks, nth, lds, st, p);
-  if (alds) return launch_lv(liouville_adjoint_kernel<true, false>, blocks, nth, lds, st, p);
-  if (galds) return launch_lv(liouville_adjoint_kernel<false, true>, blocks, nth, lds, st, p);
-  return launch_lv(liouville_adjoint_kernel<false, false>, blocks, nth, lds, st, p);
+  if (alds && galds && N == 4) return launch_lv(liouville_adjoint_kernel<true, true, 4>, blocks, nth, lds, st, p);
+  if (alds && galds && N == 9) return launch_lv(liouville_adjoint_kernel<true, true, 9>, blocks, nth, lds, st, p);
+  if (alds && galds) return launch_lv(liouville_adjoint_kernel<true, true, 0>, blocks, nth, lds, st, p);
+  if (alds) return launch_lv(liouville_adjoint_kernel<true, false, 0>, blocks, nth, lds, st, p);
+  if (galds) return launch_lv(liouville_adjoint_kernel<false, true, 0>, blocks, nth, lds, st, p);
+  return launch_lv(liouville_adjoint_kernel<false, false, 0>, blocks, nth, lds, st, p);
 }
