@@ -60,19 +60,40 @@ class Arnoldi:
         return self.H[:k + 1, :k] if not self.exhausted else self.H[:k, :k]
 
     def shifted_solve(self, shifts, rhs_sign=1.0):
-        """y_s and residual norms of (L + s) x = rhs_sign * v0 in the basis, for every shift s."""
+        """y_s and residual norms of (L + s) x = rhs_sign * v0 in the basis, for every shift s: the
+        least-squares problems min ||beta e1 - (Hbar + s Ibar) y|| of GMRES, solved for all shifts at
+        once by Givens rotations of the Hessenberg matrix (O(m^2) per shift)."""
         Hb = self.hbar()
         rows, cols = Hb.shape
+        shifts = np.atleast_1d(np.asarray(shifts, complex))
+        A = np.repeat(Hb[None].astype(complex), len(shifts), axis=0)
+        idx = np.arange(min(rows, cols))
+        A[:, idx, idx] += shifts[:, None]
+        g = np.zeros((len(shifts), rows), complex)
+        g[:, 0] = rhs_sign * self.beta
+        for j in range(min(cols, rows - 1)):
+            a, b = A[:, j, j].copy(), A[:, j + 1, j].copy()
+            r = np.sqrt(np.abs(a) ** 2 + np.abs(b) ** 2)
+            safe = np.where(r > 0, r, 1.0)
+            c = np.where(r > 0, a / safe, 1.0)
+            sn = np.where(r > 0, b / safe, 0.0)
+            top, bot = A[:, j, j:].copy(), A[:, j + 1, j:].copy()
+            A[:, j, j:] = np.conj(c)[:, None] * top + np.conj(sn)[:, None] * bot
+            A[:, j + 1, j:] = -sn[:, None] * top + c[:, None] * bot
+            gt, gb = g[:, j].copy(), g[:, j + 1].copy()
+            g[:, j] = np.conj(c) * gt + np.conj(sn) * gb
+            g[:, j + 1] = -sn * gt + c * gb
+        # back substitution on the triangular part (a zero pivot, from an exact breakdown, drops its unknown)
+        y = np.zeros((len(shifts), cols), complex)
+        for i in range(cols - 1, -1, -1):
+            acc = g[:, i] - np.einsum("sk,sk->s", A[:, i, i + 1:cols], y[:, i + 1:cols])
+            piv = A[:, i, i]
+            y[:, i] = np.where(piv != 0, acc / np.where(piv != 0, piv, 1.0), 0.0)
         e1 = np.zeros(rows, complex)
         e1[0] = rhs_sign * self.beta
-        Ib = np.eye(rows, cols)
-        ys, res = [], []
-        for s in shifts:
-            A = Hb + s * Ib
-            y = np.linalg.lstsq(A, e1, rcond=None)[0]
-            ys.append(y)
-            res.append(np.linalg.norm(e1 - A @ y))
-        return np.array(ys), np.array(res)
+        res = np.array([np.linalg.norm(e1 - (Hb @ y[q] + shifts[q] * np.concatenate([y[q], np.zeros(rows - cols)])))
+                        for q in range(len(shifts))])
+        return y, res
 
     def expm_coeffs(self, T):
         """Coefficients c of e^{L T} v0 ~ V_k c and an a-posteriori error estimate."""
