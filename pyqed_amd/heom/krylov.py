@@ -113,9 +113,10 @@ class Arnoldi:
 
 
 def grow_until(arn, check, first=16):
-    """Extend the basis until check(arn) holds or no growth is possible.  The basis doubles between
-    checks (16, 32, 64, ...): a check costs small dense solves of the basis size for every shift,
-    far more than the device work of one Arnoldi step."""
+    """Extend the basis until check(arn) holds or no growth is possible.  The basis grows by half
+    between checks (16, 24, 36, 54, ...): a check costs small dense work of the basis size (a matrix
+    exponential is cubic in it), far more than the device work of one Arnoldi step, and a smaller
+    growth factor overshoots the size the tolerance needs by less."""
     while True:
         if arn.k > 0 and check(arn):
             return arn
@@ -124,7 +125,7 @@ def grow_until(arn, check, first=16):
                 raise RuntimeError(f"Krylov basis of {arn.k} vectors did not reach the tolerance; "
                                    "raise krylov_dim or tol")
             return arn
-        arn.extend(max(first, arn.k))
+        arn.extend(max(first, arn.k // 2))
 
 
 def correlation_grid(matvec, act, trace_d, rho_vec, T, w_x, w_y, torch, tol=1e-10, m_max=400):
