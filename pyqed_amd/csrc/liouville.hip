@@ -145,11 +145,53 @@ __device__ __forceinline__ cpx<double> row_dot(const LvParams& p, const cpx<doub
   return (a0 + a1) + (a2 + a3);
 }
 
+// Compile-time N (with at most two generators, which the host guarantees for those instantiations):
+// the lane's row r = li of A_0 and A_1 lives in registers, so a product reads only x from LDS.
+template <int NT>
+struct Rows {
+  cpx<double> a[2][NT > 0 ? NT : 1];
+};
+
+template <bool ALDS, int NT>
+__device__ void load_rows(const LvParams& p, const Grp& q, const cpx<double>* sA, Rows<NT>& R) {
+  if constexpr (NT > 0) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int k = 0; k < NT; ++k)
+        R.a[i][k] = (q.live && q.li < NT && i < p.nterm) ? gen<ALDS, NT>(p, sA, i, q.li, k) : mk(0.0, 0.0);
+  }
+}
+
+template <int NT>
+__device__ __forceinline__ cpx<double> reg_dot(const cpx<double> (&row)[NT > 0 ? NT : 1], const cpx<double>* x) {
+  cpx<double> a0 = mk(0.0, 0.0), a1 = a0, a2 = a0, a3 = a0;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    if (k % 4 == 0) cmac(a0, row[k], x[k]);
+    if (k % 4 == 1) cmac(a1, row[k], x[k]);
+    if (k % 4 == 2) cmac(a2, row[k], x[k]);
+    if (k % 4 == 3) cmac(a3, row[k], x[k]);
+  }
+  return (a0 + a1) + (a2 + a3);
+}
+
 // out = A(step) x   (row r by lane r of the group, r += gsz)
 template <bool ALDS, int NT>
 __device__ void matvec(const LvParams& p, const Grp& q, const cpx<double>* sA, const double* cf,
-                       const cpx<double>* x, cpx<double>* out) {
+                       const cpx<double>* x, cpx<double>* out, const Rows<NT>& R) {
   const int N = NT > 0 ? NT : p.N;
+  if constexpr (NT > 0) {
+    if (q.live && q.li < NT) {
+      cpx<double> s = reg_dot<NT>(R.a[0], x);
+      if (p.nterm > 1) {
+        const double c = coeff(cf, 1);
+        if (c != 0.0) s = s + reg_dot<NT>(R.a[1], x) * c;
+      }
+      out[q.li] = s;
+    }
+    return;
+  }
   for (int r = q.li; q.live && r < N; r += q.gsz) {
     cpx<double> s = mk(0.0, 0.0);
     for (int i = 0; i < p.nterm; ++i) {
@@ -164,8 +206,12 @@ __device__ void matvec(const LvParams& p, const Grp& q, const cpx<double>* sA, c
 // out = A_i x for one generator
 template <bool ALDS, int NT>
 __device__ void matvec_term(const LvParams& p, const Grp& q, const cpx<double>* sA, int i, const cpx<double>* x,
-                            cpx<double>* out) {
+                            cpx<double>* out, const Rows<NT>& R) {
   const int N = NT > 0 ? NT : p.N;
+  if constexpr (NT > 0) {
+    if (q.live && q.li < NT) out[q.li] = i == 0 ? reg_dot<NT>(R.a[0], x) : reg_dot<NT>(R.a[1], x);
+    return;
+  }
   for (int r = q.li; q.live && r < N; r += q.gsz) out[r] = row_dot<ALDS, false, NT>(p, sA, i, r, x);
 }
 
@@ -222,31 +268,31 @@ __device__ void observe(const LvParams& p, const Grp& q, const cpx<double>* v, i
 // One RK4 step in place on v (LDS), field row cf.  Scratch: X, k, acc (each N).
 template <bool ALDS, int NT>
 __device__ void rk4_step(const LvParams& p, const Grp& q, const cpx<double>* sA, const double* cf, cpx<double>* v,
-                         cpx<double>* X, cpx<double>* k, cpx<double>* acc) {
+                         cpx<double>* X, cpx<double>* k, cpx<double>* acc, const Rows<NT>& R) {
   const int N = NT > 0 ? NT : p.N;
   const double dt = p.dt, dt2 = p.dt / 2.0;
-  matvec<ALDS, NT>(p, q, sA, cf, v, k);  // k1
+  matvec<ALDS, NT>(p, q, sA, cf, v, k, R);  // k1
   __syncthreads();
   for (int e = q.li; q.live && e < N; e += q.gsz) {
     acc[e] = k[e];
     X[e] = v[e] + k[e] * dt2;
   }
   __syncthreads();
-  matvec<ALDS, NT>(p, q, sA, cf, X, k);  // k2
+  matvec<ALDS, NT>(p, q, sA, cf, X, k, R);  // k2
   __syncthreads();
   for (int e = q.li; q.live && e < N; e += q.gsz) {
     acc[e] = acc[e] + k[e] * 2.0;
     X[e] = v[e] + k[e] * dt2;
   }
   __syncthreads();
-  matvec<ALDS, NT>(p, q, sA, cf, X, k);  // k3
+  matvec<ALDS, NT>(p, q, sA, cf, X, k, R);  // k3
   __syncthreads();
   for (int e = q.li; q.live && e < N; e += q.gsz) {
     acc[e] = acc[e] + k[e] * 2.0;
     X[e] = v[e] + k[e] * dt;
   }
   __syncthreads();
-  matvec<ALDS, NT>(p, q, sA, cf, X, k);  // k4
+  matvec<ALDS, NT>(p, q, sA, cf, X, k, R);  // k4
   __syncthreads();
   for (int e = q.li; q.live && e < N; e += q.gsz) {
     const cpx<double> a = acc[e] + k[e];
@@ -285,10 +331,12 @@ __global__ __launch_bounds__(kT) void liouville_forward_kernel(LvParams p) {
   double2* rho = p.rho + q.b * N;
   for (int e = q.li; q.live && e < N; e += q.gsz) v[e] = mk(rho[e].x, rho[e].y);
   __syncthreads();
+  Rows<NT> R;
+  load_rows<ALDS, NT>(p, q, sA, R);
   if (p.n_obs) observe<ALDS, NT>(p, q, v, 0, red);
   for (int64_t s = 0; s < p.nsteps; ++s) {
     if (nt1) cf_ensure(p, q, cw, s, false);
-    rk4_step<ALDS, NT>(p, q, sA, nt1 ? cw.w + (s - cw.lo) * nt1 : nullptr, v, X, k, acc);
+    rk4_step<ALDS, NT>(p, q, sA, nt1 ? cw.w + (s - cw.lo) * nt1 : nullptr, v, X, k, acc, R);
     if (p.n_obs) observe<ALDS, NT>(p, q, v, s + 1, red);
     if (p.rho_out && ((s + 1) % p.save_every) == 0) {
       const int64_t slot = (s + 1) / p.save_every - 1;
@@ -348,9 +396,11 @@ __global__ __launch_bounds__(kT) void liouville_adjoint_kernel(LvParams p) {
     ck[e] = r;
   }
   __syncthreads();
+  Rows<NT> R;
+  load_rows<ALDS, NT>(p, q, sA, R);
   for (int64_t s = 0; s < p.nsteps; ++s) {
     if (nt1) cf_ensure(p, q, cw, s, false);
-    rk4_step<ALDS, NT>(p, q, sA, nt1 ? cw.w + (s - cw.lo) * nt1 : nullptr, v, X1, k, acc);
+    rk4_step<ALDS, NT>(p, q, sA, nt1 ? cw.w + (s - cw.lo) * nt1 : nullptr, v, X1, k, acc, R);
     if ((s + 1) % ce == 0 && (s + 1) / ce < p.nck)
       for (int e = q.li; q.live && e < N; e += q.gsz) ck[((s + 1) / ce) * N + e] = make_double2(v[e].x, v[e].y);
   }
@@ -381,7 +431,7 @@ __global__ __launch_bounds__(kT) void liouville_adjoint_kernel(LvParams p) {
       __syncthreads();
       if (s + 1 < s1) {
         if (nt1) cf_ensure(p, q, cw, s, false);
-        rk4_step<ALDS, NT>(p, q, sA, nt1 ? cw.w + (s - cw.lo) * nt1 : nullptr, v, X1, k, acc);
+        rk4_step<ALDS, NT>(p, q, sA, nt1 ? cw.w + (s - cw.lo) * nt1 : nullptr, v, X1, k, acc, R);
       }
     }
     __syncthreads();
@@ -398,15 +448,15 @@ __global__ __launch_bounds__(kT) void liouville_adjoint_kernel(LvParams p) {
         X1[e] = mk(r.x, r.y);
       }
       __syncthreads();
-      matvec<ALDS, NT>(p, q, sA, cf, X1, k);
+      matvec<ALDS, NT>(p, q, sA, cf, X1, k, R);
       __syncthreads();
       for (int e = q.li; q.live && e < N; e += q.gsz) X2[e] = X1[e] + k[e] * (dt / 2.0);
       __syncthreads();
-      matvec<ALDS, NT>(p, q, sA, cf, X2, k);
+      matvec<ALDS, NT>(p, q, sA, cf, X2, k, R);
       __syncthreads();
       for (int e = q.li; q.live && e < N; e += q.gsz) X3[e] = X1[e] + k[e] * (dt / 2.0);
       __syncthreads();
-      matvec<ALDS, NT>(p, q, sA, cf, X3, k);
+      matvec<ALDS, NT>(p, q, sA, cf, X3, k, R);
       __syncthreads();
       for (int e = q.li; q.live && e < N; e += q.gsz) {
         X4[e] = X1[e] + k[e] * dt;
@@ -421,7 +471,7 @@ __global__ __launch_bounds__(kT) void liouville_adjoint_kernel(LvParams p) {
         cpx<double>* Xst = Xs[st];
         // field gradients: Re <kbar_s, A_i X_s>
         for (int i = 1; i < p.nterm; ++i) {
-          matvec_term<ALDS, NT>(p, q, sA, i, Xst, k);
+          matvec_term<ALDS, NT>(p, q, sA, i, Xst, k, R);
           __syncthreads();
           const double gi = redot(q, kb, k, N, red);
           if (q.live && q.li == 0) gw.w[(s - gw.lo) * nt1 + (i - 1)] += gi;
@@ -533,9 +583,10 @@ extern "C" int pyqed_liouville_run(int32_t N, int32_t nterm, const void* A, int3
   const size_t lds = (alds ? a_bytes : 0) + st_bytes;
   if (lds > maxl) return fail(PYQED_EINVAL, "liouville_run: the state does not fit one workgroup's LDS");
   hipStream_t st = (hipStream_t)stream;
-  // N = 4 (a qubit) and N = 9 (three levels) at compile time: the row products unroll fully
-  if (alds && N == 4) return launch_lv(liouville_forward_kernel<true, 4>, blocks, nth, lds, st, p);
-  if (alds && N == 9) return launch_lv(liouville_forward_kernel<true, 9>, blocks, nth, lds, st, p);
+  // N = 4 (a qubit) and N = 9 (three levels) at compile time, at most two generators: the lane's
+  // generator rows in registers, the row products unrolled
+  if (alds && nterm <= 2 && N == 4) return launch_lv(liouville_forward_kernel<true, 4>, blocks, nth, lds, st, p);
+  if (alds && nterm <= 2 && N == 9) return launch_lv(liouville_forward_kernel<true, 9>, blocks, nth, lds, st, p);
   return alds ? launch_lv(liouville_forward_kernel<true, 0>, blocks, nth, lds, st, p)
               : launch_lv(liouville_forward_kernel<false, 0>, blocks, nth, lds, st, p);
 }
@@ -590,8 +641,10 @@ extern "C" int pyqed_liouville_adjoint(int32_t N, int32_t nterm, const void* A, 
   const size_t lds = (alds ? a_bytes : 0) + st2;
   if (lds > maxl) return fail(PYQED_EINVAL, "liouville_adjoint: the state does not fit one workgroup's LDS");
   hipStream_t st = (hipStream_t)stream;
-  if (alds && galds && N == 4) return launch_lv(liouville_adjoint_kernel<true, true, 4>, blocks, nth, lds, st, p);
-  if (alds && galds && N == 9) return launch_lv(liouville_adjoint_kernel<true, true, 9>, blocks, nth, lds, st, p);
+  if (alds && galds && nterm <= 2 && N == 4)
+    return launch_lv(liouville_adjoint_kernel<true, true, 4>, blocks, nth, lds, st, p);
+  if (alds && galds && nterm <= 2 && N == 9)
+    return launch_lv(liouville_adjoint_kernel<true, true, 9>, blocks, nth, lds, st, p);
   if (alds && galds) return launch_lv(liouville_adjoint_kernel<true, true, 0>, blocks, nth, lds, st, p);
   if (alds) return launch_lv(liouville_adjoint_kernel<true, false, 0>, blocks, nth, lds, st, p);
   if (galds) return launch_lv(liouville_adjoint_kernel<false, true, 0>, blocks, nth, lds, st, p);
