@@ -48,6 +48,20 @@ def test_lindblad_driven(legacy):
     assert np.abs(res.observables - ref["observables"]).max() < 1e-12
 
 
+@pytest.mark.parametrize("d,nfield", [(2, 1), (3, 2), (2, 3)])
+def test_lindblad_driven_field_counts(d, nfield):
+    # one field: the compile-time N = 4 / 9 kernels keep the generator rows in registers; more than
+    # one field takes the general kernel
+    from pyqed_amd.oqs import LindbladSolver
+    print(f"[case] driven Lindblad d={d} fields={nfield}", flush=True)
+    H, c, rho0, e_ops = _system(d, seed=5 + d)
+    rng = np.random.default_rng(6)
+    terms = [[_rand_herm(d, rng), (lambda t, w=1.0 + 0.4 * j: 0.2 * np.cos(w * t))] for j in range(nfield)]
+    res = LindbladSolver([H] + terms, c).run(rho0, 0.02, 250, e_ops=e_ops)
+    ref = OL.run_driven([H] + terms, rho0, c, 0.02, 250, e_ops=e_ops)
+    assert np.abs(res.observables - ref["observables"]).max() < 1e-12
+
+
 def test_lindblad_batch_of_initial_states():
     from pyqed_amd.oqs import LindbladSolver
     H, c, _, _ = _system(4, seed=3)
