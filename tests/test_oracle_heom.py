@@ -49,7 +49,7 @@ def test_rank_is_tier_ordered_bijection_brute_force(K, L):
             assert tuple(keys[r]) == n
 
 
-def test_known_answers_of_survey():
+def test_known_answers_combinadic_order():
     assert [tuple(k) for k in O.gen_keys(2, 5)[:11]] == [(0, 0), (0, 1), (1, 0), (0, 2), (1, 1), (2, 0), (0, 3),
                                                            (1, 2), (2, 1), (3, 0), (0, 4)]
     assert [tuple(k) for k in O.gen_keys(3, 2)] == [(0, 0, 0), (0, 0, 1), (0, 1, 0), (1, 0, 0), (0, 0, 2), (0, 1, 1),
