@@ -44,8 +44,7 @@ def lookup(engine, precision, path=TUNING_FILE):
 
 
 def set_tiling(engine, target, bpc, variant=0):
-    _lib.check(engine.L.pyqed_heom_plan_set_tiling(engine.plan, int(target), int(bpc)))
-    _lib.check(engine.L.pyqed_heom_plan_set_variant(engine.plan, int(variant)))
+    engine.set_tiling(target, bpc, variant)
 
 
 def tune(engine, dt, precision="mixed", targets=TARGETS, blocks_per_cu=BLOCKS_PER_CU, variants=VARIANTS, steps=2,

@@ -250,11 +250,12 @@ def decompose_spectrum_pade_imag(spe, w_sp):
     return decompose_spectrum_pade_real(spe, w_sp)
 
 
-def prony_find_gamma(h, n_sample, nind):
+def prony_find_gamma(h, n_sample, nind, *, verbose=False):
     """Prony roots (deom.py:427-444): eigen-decompose the n_sample x n_sample Hankel matrix of h
     (scipy.linalg.eig), rotate each eigenvector by exp(-i arg(sigma)/2), sort by |sigma| descending,
     take the roots of the polynomial whose coefficients are the entries of vector number nind
-    (highest degree last) and keep the nind roots of smallest modulus."""
+    (highest degree last) and keep the nind roots of smallest modulus.  verbose=True prints nind, as the
+    reference does on every call."""
     import scipy.linalg as la
 
     mat_h = np.zeros((n_sample, n_sample))
@@ -266,24 +267,27 @@ def prony_find_gamma(h, n_sample, nind):
     Qp = np.dot(Q, phase_mat)
     order = np.argsort(vs)[::-1]
     Qp = Qp[:, order]
-    print(nind)
+    if verbose:
+        print(nind)
     gamma = np.roots(Qp[:, nind][::-1])
     return gamma[np.argsort(np.abs(gamma))[:nind]]
 
 
-def prony_fitting(h, t, nind, scale, n, gamma_real=None, gamma_imag=None):
+def prony_fitting(h, t, nind, scale, n, gamma_real=None, gamma_imag=None, *, verbose=False):
     """Least-squares amplitudes for Prony roots (deom.py:447-504): gamma from prony_find_gamma on Re h
     (nind an int) or separately on Re h and Im h (nind = [n_real, n_imag]) unless given; the normal
     equations of the stacked real system; expn = -2 n log(gamma) / scale; then sort_symmetry."""
     import scipy.linalg as la
 
     if isinstance(nind, list):
-        gamma_real = prony_find_gamma(np.real(h), n, nind[0]) if gamma_real is None else np.array(gamma_real)
-        gamma_imag = prony_find_gamma(np.imag(h), n, nind[1]) if gamma_imag is None else np.array(gamma_imag)
+        gamma_real = (prony_find_gamma(np.real(h), n, nind[0], verbose=verbose) if gamma_real is None
+                      else np.array(gamma_real))
+        gamma_imag = (prony_find_gamma(np.imag(h), n, nind[1], verbose=verbose) if gamma_imag is None
+                      else np.array(gamma_imag))
         gamma = np.append(gamma_real, gamma_imag)
         n_row = nind[0] + nind[1]
     else:
-        gamma = prony_find_gamma(np.real(h), n, nind)
+        gamma = prony_find_gamma(np.real(h), n, nind, verbose=verbose)
         n_row = nind
     t_new = 2 * n * np.log(gamma)
     n_col = len(t)
@@ -302,37 +306,37 @@ def prony_fitting(h, t, nind, scale, n, gamma_real=None, gamma_imag=None):
     return sort_symmetry(etal_p, expn_p)
 
 
-def decompose_spectrum_prony(spe, w_sp, beta, nind, scale=250000, n=1250, npsd=10, bose_fermi=1):
+def decompose_spectrum_prony(spe, w_sp, beta, nind, scale=250000, n=1250, npsd=10, bose_fermi=1, *, verbose=False):
     """Prony decomposition of a sympy spectral density (deom.py:507-542): sample C(t) from its Pade
     decomposition (npsd poles) at scale * linspace(0, 1, 2n + 1) and fit nind exponentials.  With
     nind = ['a', k] (or [k, 'a']) the real (imaginary) part keeps its analytic exponentials; the
     reference ends the program for the Bose real-part case and the Fermi imaginary-part case, and so
-    does this function (SystemExit)."""
+    does this function (SystemExit, with the reference's message).  verbose=True prints the first and
+    last ten samples and the roots' counts, as the reference always does."""
     etal_pade, _, _, expn_pade = decompose_spectrum_pade(spe, w_sp, beta, npsd, bose_fermi=bose_fermi)
     t = np.linspace(0, 1, 2 * n + 1)
     res_t = np.zeros(len(t), dtype=complex)
     fit_t(scale * t, res_t, expn_pade, etal_pade)
-    print("check the sample points")
-    print(res_t[:10])
-    print(res_t[-10:])
+    if verbose:
+        print("check the sample points")
+        print(res_t[:10])
+        print(res_t[-10:])
     if isinstance(nind, list):
         if nind[0] == 'a':
             _, _, _, expn_real = decompose_spectrum_pade_real(spe, w_sp)
             gamma_real = np.exp(-expn_real * scale / (2 * n))
             nind[0] = len(gamma_real)
             if bose_fermi == 1:
-                print("For the bose case, C(t) have the analytical imag part")
-                raise SystemExit
-            return prony_fitting(res_t, t, nind, scale, n, gamma_real=gamma_real)
+                raise SystemExit("For the bose case, C(t) have the analytical imag part")
+            return prony_fitting(res_t, t, nind, scale, n, gamma_real=gamma_real, verbose=verbose)
         elif nind[1] == 'a':
             _, _, _, expn_imag = decompose_spectrum_pade_imag(spe, w_sp)
             gamma_imag = np.exp(-expn_imag * scale / (2 * n))
             nind[1] = len(gamma_imag)
             if bose_fermi == 2:
-                print("For the fermi case, C(t) have the analytical real part")
-                raise SystemExit
-            return prony_fitting(res_t, t, nind, scale, n, gamma_imag=gamma_imag)
-    return prony_fitting(res_t, t, nind, scale, n)
+                raise SystemExit("For the fermi case, C(t) have the analytical real part")
+            return prony_fitting(res_t, t, nind, scale, n, gamma_imag=gamma_imag, verbose=verbose)
+    return prony_fitting(res_t, t, nind, scale, n, verbose=verbose)
 
 
 def single_oscillator(omega, w_sp, beta, nind):
@@ -384,7 +388,8 @@ class Bath:
             function = [decompose_spectrum_pade] * len(spectrum_sp)
         if isinstance(spectrum_sp, list) and isinstance(beta, list) and isinstance(npsd, list):
             if len(spectrum_sp) != len(beta) or len(spectrum_sp) != len(npsd):
-                print("the length of bath, w_sp, beta, npsd is not equal!")
+                import warnings
+                warnings.warn("the length of bath, w_sp, beta, npsd is not equal!")
             parts = []
             for i in range(len(spectrum_sp)):
                 f = function[i]
@@ -397,10 +402,8 @@ class Bath:
             self.etaa = np.concatenate([np.asarray(p[2]) for p in parts]).astype(np.complex128)
             self.expn = np.concatenate([np.asarray(p[3]) for p in parts]).astype(np.complex128)
             if self.mode is None:
-                print("mode is not set!")
                 raise ValueError("mode is not set")
             if len(self.mode) != len(self.expn):
-                print("the length of mode is not equal to the number of dissipatons!")
                 raise ValueError("the length of mode is not equal to the number of dissipatons")
         else:
             # The reference's single-expression branch is unreachable (len() of a sympy

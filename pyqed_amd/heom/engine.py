@@ -201,8 +201,27 @@ class HeomEngine:
         torch = self.torch
         need = self.ws_bytes[prec]
         if self._ws is None or self._ws.numel() < need:
+            # captured graphs hold the old workspace's address: drop them before it is released
+            self._drop_graphs()
             self._ws = torch.empty(max(need, 256), dtype=torch.uint8, device=self.device)
         return self._ws
+
+    def _drop_graphs(self):
+        """Forget every captured step graph.  A graph holds raw device addresses of the workspace and of
+        the plan's work queue, so it must not outlive either (workspace growth, a tiling or variant
+        change, close())."""
+        g = getattr(self, "_graphs", None)
+        if g:
+            self.torch.cuda.synchronize()
+            g.clear()
+
+    def set_tiling(self, target_elems, blocks_per_cu, variant=0):
+        """Work-item size, persistent blocks per CU and kernel variant of the streamed stage kernels.
+        Rebuilding the work queue frees the old one, so captured graphs are dropped first."""
+        self._drop_graphs()
+        _lib.check(self.L.pyqed_heom_plan_set_tiling(self.plan, int(target_elems), int(blocks_per_cu)))
+        _lib.check(self.L.pyqed_heom_plan_set_variant(self.plan, int(variant)))
+        self.variant = int(variant)
 
     def set_state(self, states):
         """Load full hierarchy states (list of [nmax, d, d] arrays in the caller's basis) into y."""
@@ -257,13 +276,20 @@ class HeomEngine:
         [>= step+nsteps, 6]; `traj` a device complex128 tensor [nsteps//save_every, traj_stride]."""
         torch = self.torch
         prec = PRECISIONS[precision]
+        if nsteps < 0 or save_every < 1:
+            raise ValueError("advance: nsteps must be >= 0 and save_every >= 1")
+        if field is not None:
+            self._check_tensor("field", field, torch.float64, (self.step + nsteps, 6))
+        elif self.driven:
+            raise ValueError("advance: an instance has a system or coupling dipole; pass its field table")
+        if traj is not None:
+            self._check_tensor("traj", traj, torch.complex128, (nsteps // save_every, self.traj_stride))
         if self.autotune_cache and precision not in self._tuned:
             from . import autotune
             self._tuned.add(precision)
             hit = autotune.lookup(self, precision)
             if hit is not None:
-                autotune.set_tiling(self, *hit)
-                self.variant = hit[2]
+                self.set_tiling(*hit)
         ws = self.workspace(prec)
         _lib.check(self.L.pyqed_heom_run(
             self.plan, prec, self.y.data_ptr(), ws.data_ptr(), ws.numel(),
@@ -271,21 +297,39 @@ class HeomEngine:
             None if traj is None else traj.data_ptr(), save_every, _lib.stream_ptr(torch, stream)))
         self.step += nsteps
 
+    def _check_tensor(self, name, t, dtype, min_shape):
+        """t must be a contiguous tensor of `dtype` on this engine's device with at least min_shape[0] rows
+        of exactly min_shape[1] columns: the kernels index it by step and column with no bounds of their own."""
+        ok = (isinstance(t, self.torch.Tensor) and t.dtype == dtype and t.device == self.device and t.is_contiguous()
+              and t.dim() == 2 and t.shape[0] >= min_shape[0] and t.shape[1] == min_shape[1])
+        if not ok:
+            got = (tuple(t.shape), t.dtype, t.device, t.is_contiguous()) if isinstance(t, self.torch.Tensor) else type(t)
+            raise ValueError(f"advance: {name} must be a contiguous {dtype} tensor on {self.device} of shape "
+                             f"[>= {min_shape[0]}, {min_shape[1]}]; got {got}")
+
+    @property
+    def driven(self):
+        """True when some instance has a system or coupling dipole (its kernels read the field table)."""
+        return any(p["sdip"] is not None or p["Cmat"] is not None for p in self.prep)
+
     def advance_graph(self, nsteps, dt, precision="fp64", chunk=25):
         """Advance nsteps steps by replaying a captured HIP graph of `chunk` steps (the stage
         launches of every kernel class, and the LDS-resident launch on its side stream).  Without a
         time-dependent field no kernel reads the step index, so one captured chunk serves every
         replay.  The remainder runs eagerly.  Returns the number of graph replays."""
         torch = self.torch
-        key = (precision, chunk, float(dt))
+        if self.driven:
+            raise ValueError("advance_graph: a captured chunk replays one step index, so instances with a "
+                             "system or coupling dipole (time-dependent fields) must use advance()")
         if getattr(self, "_graphs", None) is None:
             self._graphs = {}
+        self.workspace(PRECISIONS[precision])
+        if self.autotune_cache and precision not in self._tuned:
+            self.advance(0, dt, precision)  # apply the cached tiling before looking up or capturing a graph
+        key = (precision, chunk, float(dt))
         g = self._graphs.get(key)
         if g is None:
             step0 = self.step
-            self.workspace(PRECISIONS[precision])
-            if self.autotune_cache and precision not in self._tuned:
-                self.advance(0, dt, precision)  # apply cached tiling before capture
             y0 = self.y.clone()
             s = torch.cuda.Stream()
             s.wait_stream(torch.cuda.current_stream())
@@ -372,6 +416,7 @@ class HeomEngine:
 
     def close(self):
         if getattr(self, "plan", None):
+            self._drop_graphs()
             self.L.pyqed_heom_plan_destroy(self.plan)
             self.plan = None
 

@@ -23,6 +23,9 @@ Fixtures written (all under tests/golden/):
                  for the spin-boson case (1000 steps), the 7-site exciton
                  (20 steps), the sigma_x-coupled two-level example, a driven
                  case, and small seeded batched-workload instances.
+  traj_cdip.npz  time-dependent couplings Q_m(t) = Q_m + coupling_dipole[m] f_c(t)
+                 (deom.py:676-687): diagonal, non-commuting and 2-D (row-broadcast) dipoles.
+  traj_dense.npz couplings with no common eigenbasis (deom.py:651-664 with general Q[m]).
 """
 import argparse
 import hashlib
@@ -278,6 +281,115 @@ def traj_c5_onestep(ref):
          **{"bath_" + k: v for k, v in bath_arrays(bath).items()})
 
 
+def drude_bath_ref(ref, params, npsd=1):
+    """Reference Bath of one Drude spectrum per (lam, gam, beta) in params, mode = bath index."""
+    import sympy as sp
+    wsym = sp.symbols(r"\omega", real=True)
+    spes = [(2 * lam * gam * wsym / (gam ** 2 + wsym ** 2)) for lam, gam, _ in params]
+    mode = list(np.repeat(np.arange(len(params)), npsd + 1))
+    return ref.Bath(spes, wsym, [b for _, _, b in params], [npsd] * len(params), mode)
+
+
+def traj_cdip(ref):
+    """Time-dependent couplings Q_m(t) = Q_m + coupling_dipole[m] f_c(t) (deom.py:676-687).
+
+    cdip_diag   d = 2: coupling sz, coupling dipole 0.5 sz (3-D, one matrix per coupling), f_c =
+                0.4 cos(1.5 t), together with a system dipole sx driven by 0.25 sin(t).
+    cdip_dense  d = 2: coupling sz, coupling dipole 0.3 sx: Q(t) no longer commutes with Q(0).
+    cdip_rows   d = 3: one coupling, a 2-D coupling_dipole, of which generate_time adds row m to every
+                row of Q_m (numpy broadcasting), so Q(t) is not Hermitian.
+    cdip_sites  d = 6: site projectors with one Drude bath each and coupling dipoles 0.3 |j><j|."""
+    _, sx, _, sz = pauli()
+    out = {}
+    fc = lambda t: 0.4 * np.cos(1.5 * t)  # noqa: E731
+    fs = lambda t: 0.25 * np.sin(t)  # noqa: E731
+    rho0 = np.array([[0.6, 0.2 - 0.1j], [0.2 + 0.1j, 0.4]], np.complex128)
+    bath = drude_bath_ref(ref, [(0.2, 0.7, 1.0)])
+    H = 0.5 * sz + 0.3 * sx
+    t, traj, _ = run_solver(ref, H, [sz], bath, 4, rho0, 0.02, 200, sdip=sx, fs=fs, cdip=np.array([0.5 * sz]), fc=fc)
+    out["cdip_diag"] = dict(H=H, Q=np.array([sz]), sdip=sx, cdip=np.array([0.5 * sz]), rho0=rho0, dt=np.array(0.02),
+                            nt=np.array(200), L=np.array(4), rho=traj, fs=np.array([0.25, 1.0]), fc=np.array([0.4, 1.5]),
+                            **{"bath_" + k: v for k, v in bath_arrays(bath).items()})
+    t, traj, _ = run_solver(ref, H, [sz], bath, 4, rho0, 0.02, 200, cdip=np.array([0.3 * sx]), fc=fc)
+    out["cdip_dense"] = dict(H=H, Q=np.array([sz]), cdip=np.array([0.3 * sx]), rho0=rho0, dt=np.array(0.02),
+                             nt=np.array(200), L=np.array(4), rho=traj, fc=np.array([0.4, 1.5]),
+                             **{"bath_" + k: v for k, v in bath_arrays(bath).items()})
+    H3 = np.array([[0.5, 0.2, 0.0], [0.2, 0.0, 0.3], [0.0, 0.3, -0.4]], np.complex128)
+    Q3 = np.diag([1.0, 0.0, -1.0]).astype(np.complex128)
+    C2 = np.array([[0.2, 0.1, 0.0], [0.1, -0.1, 0.05], [0.0, 0.05, 0.3]], np.complex128)
+    r3 = np.diag([0.5, 0.3, 0.2]).astype(np.complex128)
+    r3[0, 1], r3[1, 0] = 0.1, 0.1
+    bath3 = drude_bath_ref(ref, [(0.15, 0.9, 1.2)])
+    t, traj, _ = run_solver(ref, H3, [Q3], bath3, 3, r3, 0.02, 150, cdip=C2, fc=fc)
+    out["cdip_rows"] = dict(H=H3, Q=np.array([Q3]), cdip=C2, rho0=r3, dt=np.array(0.02), nt=np.array(150),
+                            L=np.array(3), rho=traj, fc=np.array([0.4, 1.5]),
+                            **{"bath_" + k: v for k, v in bath_arrays(bath3).items()})
+    d = 6
+    rng = np.random.default_rng(61)
+    H6 = np.diag(rng.normal(0, 0.3, d)) + np.diag(np.full(d - 1, -0.4), 1) + np.diag(np.full(d - 1, -0.4), -1)
+    H6 = H6.astype(np.complex128)
+    Q6 = np.array([np.diag(np.eye(d)[j]) for j in range(d)], np.complex128)
+    C6 = 0.3 * Q6
+    r6 = np.zeros((d, d), np.complex128)
+    r6[0, 0] = 1.0
+    bath6 = drude_bath_ref(ref, [(0.1, 0.8, 1.0)] * d)
+    t0 = time.time()
+    t, traj, _ = run_solver(ref, H6, list(Q6), bath6, 2, r6, 0.02, 100, cdip=C6, fc=fc)
+    print(f"cdip_sites {time.time()-t0:.1f}s", flush=True)
+    out["cdip_sites"] = dict(H=H6, Q=Q6, cdip=C6, rho0=r6, dt=np.array(0.02), nt=np.array(100), L=np.array(2),
+                             rho=traj, fc=np.array([0.4, 1.5]), **{"bath_" + k: v for k, v in bath_arrays(bath6).items()})
+    save("traj_cdip.npz", **{f"{c}__{k}": v for c, arrs in out.items() for k, v in arrs.items()})
+
+
+def traj_dense(ref):
+    """Couplings that do not commute, so no common eigenbasis exists (deom.py:651-664 with general Q[m]).
+
+    dense_sxsz  d = 2: sx on bath 0 and sz on bath 1 (mode = [0, 0, 1, 1]).
+    dense_three d = 3: a tridiagonal hopping operator and diag(1, 0, -1), two baths, complex H.
+    dense_six   d = 6: nearest-neighbour bond operators |j><j+1| + h.c. on three baths, site 0 projector
+                on a fourth."""
+    _, sx, _, sz = pauli()
+    out = {}
+    H = 0.5 * sz + 0.25 * sx
+    rho0 = np.array([[1, 0], [0, 0]], np.complex128)
+    bath = drude_bath_ref(ref, [(0.1, 0.5, 1.0), (0.2, 0.9, 1.5)])
+    t0 = time.time()
+    t, traj, _ = run_solver(ref, H, [sx, sz], bath, 4, rho0, 0.01, 300)
+    print(f"dense_sxsz {time.time()-t0:.1f}s", flush=True)
+    out["dense_sxsz"] = dict(H=H, Q=np.array([sx, sz]), rho0=rho0, dt=np.array(0.01), nt=np.array(300), L=np.array(4),
+                             rho=traj, **{"bath_" + k: v for k, v in bath_arrays(bath).items()})
+    H3 = np.array([[0.4, 0.2 - 0.1j, 0.0], [0.2 + 0.1j, 0.0, 0.25], [0.0, 0.25, -0.3]], np.complex128)
+    Qa = 0.5 * np.array([[0, 1, 0], [1, 0, 1], [0, 1, 0]], np.complex128)
+    Qb = np.diag([1.0, 0.0, -1.0]).astype(np.complex128)
+    r3 = np.diag([0.2, 0.5, 0.3]).astype(np.complex128)
+    bath3 = drude_bath_ref(ref, [(0.12, 0.6, 1.0), (0.08, 1.1, 0.8)])
+    t0 = time.time()
+    t, traj, _ = run_solver(ref, H3, [Qa, Qb], bath3, 3, r3, 0.01, 300)
+    print(f"dense_three {time.time()-t0:.1f}s", flush=True)
+    out["dense_three"] = dict(H=H3, Q=np.array([Qa, Qb]), rho0=r3, dt=np.array(0.01), nt=np.array(300), L=np.array(3),
+                              rho=traj, **{"bath_" + k: v for k, v in bath_arrays(bath3).items()})
+    d = 6
+    rng = np.random.default_rng(62)
+    H6 = np.diag(rng.normal(0, 0.3, d)) + np.diag(np.full(d - 1, -0.5), 1) + np.diag(np.full(d - 1, -0.5), -1)
+    H6 = H6.astype(np.complex128)
+    Q6 = []
+    for j in (0, 2, 4):
+        b = np.zeros((d, d), np.complex128)
+        b[j, j + 1] = b[j + 1, j] = 1.0
+        Q6.append(b)
+    Q6.append(np.diag(np.eye(d)[0]).astype(np.complex128))
+    Q6 = np.array(Q6)
+    r6 = np.zeros((d, d), np.complex128)
+    r6[0, 0] = 1.0
+    bath6 = drude_bath_ref(ref, [(0.1, 0.8, 1.0)] * 4)
+    t0 = time.time()
+    t, traj, _ = run_solver(ref, H6, list(Q6), bath6, 2, r6, 0.02, 100)
+    print(f"dense_six {time.time()-t0:.1f}s", flush=True)
+    out["dense_six"] = dict(H=H6, Q=Q6, rho0=r6, dt=np.array(0.02), nt=np.array(100), L=np.array(2), rho=traj,
+                            **{"bath_" + k: v for k, v in bath_arrays(bath6).items()})
+    save("traj_dense.npz", **{f"{c}__{k}": v for c, arrs in out.items() for k, v in arrs.items()})
+
+
 # --------------------------------------------------------------------------- Prony route
 def gen_prony(ref):
     """decompose_spectrum_prony (deom.py:507-542) on a Drude spectrum (nind = 3, one fit of Re C) and a
@@ -327,6 +439,7 @@ def gen_prony(ref):
 JOBS = {
     "baths": gen_baths, "tables": gen_tables, "c1": traj_c1, "sx": traj_example_sx,
     "driven": traj_driven, "c2": traj_c2, "c3": traj_c3_small, "c5": traj_c5_onestep, "prony": gen_prony,
+    "cdip": traj_cdip, "dense": traj_dense,
 }
 
 if __name__ == "__main__":

@@ -39,3 +39,73 @@ def test_graph_replays_match_eager(prec):
     graph.advance_graph(7, 0.01, prec, chunk=chunk)
     torch.cuda.synchronize()
     assert torch.equal(eager.y, graph.y)
+
+
+def test_graph_recaptured_after_tiling_and_workspace_change():
+    """A captured chunk holds the addresses of the workspace and of the work queue.  Changing the tiling
+    (which rebuilds the queue) or running another precision (which can grow the workspace) must drop it,
+    and later replays must still equal eager steps."""
+    import torch
+    from pyqed_amd.heom.engine import HeomEngine
+    insts = _insts()
+    eager = HeomEngine(insts)
+    graph = HeomEngine(insts)
+    for e in (eager, graph):
+        e.autotune_cache = False
+    chunk = 4
+
+    def both(fn_eager, fn_graph):
+        fn_eager()
+        fn_graph()
+        torch.cuda.synchronize()
+        assert torch.equal(eager.y, graph.y)
+
+    both(lambda: eager.advance(chunk, 0.01, "mixed"), lambda: graph.advance_graph(chunk, 0.01, "mixed", chunk=chunk))
+    assert len(graph._graphs) == 1
+    # another precision: the FP64 workspace is larger, so the mixed graph's workspace is replaced
+    from pyqed_amd import _lib
+    grows = graph.ws_bytes[_lib.PREC_FP64] > graph.ws_bytes[_lib.PREC_MIXED]
+    both(lambda: eager.advance(3, 0.01, "fp64"), lambda: graph.advance(3, 0.01, "fp64"))
+    assert len(graph._graphs) == (0 if grows else 1)
+    both(lambda: eager.advance(chunk, 0.01, "mixed"), lambda: graph.advance_graph(chunk, 0.01, "mixed", chunk=chunk))
+    # a new tiling rebuilds the work queue
+    for e in (eager, graph):
+        e.set_tiling(2048, 4, 0)
+    assert len(graph._graphs) == 0
+    for _ in range(2):
+        both(lambda: eager.advance(chunk, 0.01, "mixed"),
+             lambda: graph.advance_graph(chunk, 0.01, "mixed", chunk=chunk))
+    graph.close()
+    assert not graph._graphs
+
+
+def test_advance_rejects_bad_field_and_traj():
+    """The kernels index the field table by step and the trajectory by saved row with no bounds of their
+    own, so advance() checks both; graph capture refuses driven instances."""
+    import torch
+    from pyqed_amd.heom.engine import HeomEngine
+    sx = np.array([[0, 1], [1, 0]], complex)
+    inst = dict(H=np.diag([0.5, -0.5]).astype(complex), Q=np.array([np.diag([1.0, -1.0])]), L=2,
+                rho0=np.diag([1.0, 0.0]).astype(complex), sdip=sx,
+                bath=(np.array([0.1 + 0j]), np.array([0.1 + 0j]), np.array([0.1 + 0j]), np.array([1.0 + 0j]),
+                      np.array([0])))
+    eng = HeomEngine([inst])
+    dev = eng.device
+    with pytest.raises(ValueError):
+        eng.advance(5, 0.01, "fp64")  # driven, no field
+    with pytest.raises(ValueError):
+        eng.advance(5, 0.01, "fp64", field=torch.zeros((4, 6), dtype=torch.float64, device=dev))  # too short
+    with pytest.raises(ValueError):
+        eng.advance(5, 0.01, "fp64", field=torch.zeros((5, 6), dtype=torch.float32, device=dev))
+    with pytest.raises(ValueError):
+        eng.advance(5, 0.01, "fp64", field=torch.zeros((5, 6), dtype=torch.float64))  # host tensor
+    f = torch.zeros((10, 6), dtype=torch.float64, device=dev)
+    eng.advance(5, 0.01, "fp64", field=f)
+    with pytest.raises(ValueError):
+        eng.advance(6, 0.01, "fp64", field=f)  # rows 5..10 needed after 5 steps
+    with pytest.raises(ValueError):
+        eng.advance(5, 0.01, "fp64", field=f, traj=torch.zeros((2, eng.traj_stride), dtype=torch.complex128,
+                                                              device=dev))
+    with pytest.raises(ValueError):
+        eng.advance_graph(5, 0.01, "fp64", chunk=5)
+    torch.cuda.synchronize()

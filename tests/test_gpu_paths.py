@@ -159,15 +159,13 @@ def test_pipelined_variant_bitwise_equal(stage_prec):
     """The two-deep pipelined kernel computes the same arithmetic in the same order as the base
     projector kernel: identical states after several steps, across many work items."""
     import torch
-    from pyqed_amd import _lib
     from pyqed_amd.heom.engine import HeomEngine
     insts = [_instance(d, L, seed=200 + d) for d in (12, 14, 16) for L in (2, 3)]
     outs = []
     for variant in (0, 1):
         eng = HeomEngine(insts, resident=False, packed=False)  # the pipelined kernel has no packed form
         eng.autotune_cache = False
-        _lib.check(eng.L.pyqed_heom_plan_set_tiling(eng.plan, 1024, 4))
-        _lib.check(eng.L.pyqed_heom_plan_set_variant(eng.plan, variant))
+        eng.set_tiling(1024, 4, variant)
         assert eng.hermitian and eng.classes()["items"][2] > 0
         eng.advance(3, 0.02, stage_prec)
         torch.cuda.synchronize()
