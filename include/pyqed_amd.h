@@ -262,6 +262,19 @@ int pyqed_spectral_sum(int32_t kind, int32_t nrow, int32_t k, int32_t ncol, int3
                        const void* lam, const void* B, const double* s, void* out, int64_t so_t, int64_t so_a,
                        int64_t so_b, void* work, int64_t work_bytes, void* stream);
 
+/* pyqed_spectral_sum without a workspace argument and without any device allocation: no counterpart in
+ * the reference.  It is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The weights are staged in a static device
+ * array of PYQED_SPECTRAL_STATIC_MAX complex doubles (16 MiB), one per GPU, held until the process
+ * ends; k * ns above that limit returns PYQED_EINVAL.  Launches go to `stream`, which must belong to
+ * the current device.  Every call on one GPU shares that GPU's array: calls that overlap in time on
+ * one GPU (different streams, or host threads) race and must be serialised by the caller.  Results
+ * are bitwise those of pyqed_spectral_sum (same kernels). */
+#define PYQED_SPECTRAL_STATIC_MAX (1 << 20)
+int pyqed_spectral_sum_static(int32_t kind, int32_t nrow, int32_t k, int32_t ncol, int32_t ns, const void* A,
+                              const void* lam, const void* B, const double* s, void* out, int64_t so_t,
+                              int64_t so_a, int64_t so_b, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,8 +27,9 @@ SYMBOLS = [
     "pyqed_heom_plan_classes", "pyqed_heom_plan_set_tiling", "pyqed_heom_plan_get_tiling", "pyqed_heom_plan_set_variant",
     "pyqed_heom_plan_class_state", "pyqed_heom_plan_set_timing", "pyqed_heom_plan_read_timing",
     "pyqed_liouville_run", "pyqed_liouville_adjoint", "pyqed_legacy_heom_run",
-    "pyqed_spectral_work_bytes", "pyqed_spectral_sum",
+    "pyqed_spectral_work_bytes", "pyqed_spectral_sum", "pyqed_spectral_sum_static",
 ]
+SPECTRAL_STATIC_MAX = 1 << 20  # complex weights the static scratch of pyqed_spectral_sum_static holds
 
 
 class PyqedError(RuntimeError):
@@ -86,6 +87,7 @@ def lib():
     L.pyqed_legacy_heom_run.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, vp, i64, dbl, i32, vp, vp, vp]
     L.pyqed_spectral_work_bytes.argtypes = [i32, i32]
     L.pyqed_spectral_sum.argtypes = [i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, i64, i64, i64, vp, i64, vp]
+    L.pyqed_spectral_sum_static.argtypes = [i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, i64, i64, i64, vp]
     for name in SYMBOLS:
         if name != "pyqed_last_error":
             getattr(L, name).restype = C.c_int

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "common.h"
@@ -208,27 +209,20 @@ extern "C" int64_t pyqed_spectral_work_bytes(int32_t k, int32_t ns) {
   return (int64_t)k * ns * (int64_t)sizeof(double2);
 }
 
-extern "C" int pyqed_spectral_sum(int32_t kind, int32_t nrow, int32_t k, int32_t ncol, int32_t ns, const void* A,
-                                  const void* lam, const void* B, const double* s, void* out, int64_t so_t,
-                                  int64_t so_a, int64_t so_b, void* work, int64_t work_bytes, void* stream) {
-  if (kind != 0 && kind != 1) return fail(PYQED_EINVAL, "spectral_sum: kind must be 0 (exp) or 1 (resolvent)");
-  if (nrow < 1 || k < 1 || ncol < 1 || ns < 1)
-    return fail(PYQED_EINVAL, "spectral_sum: nrow, k, ncol and ns must be positive");
-  if (nrow > 65535) return fail(PYQED_EINVAL, "spectral_sum: at most 65535 rows per call");
-  if (!A || !lam || !B || !s || !out || !work) return fail(PYQED_EINVAL, "spectral_sum: null pointer");
-  if (work_bytes < pyqed_spectral_work_bytes(k, ns))
-    return fail(PYQED_EINVAL, "spectral_sum: workspace of " + std::to_string(work_bytes) + " bytes < " +
-                                  std::to_string(pyqed_spectral_work_bytes(k, ns)) + " needed");
-  hipStream_t st = (hipStream_t)stream;
-  double2* W = (double2*)work;
+namespace {
+
+// Both launches of one spectral sum, with the weights staged in W (>= k * ns complex doubles).
+int launch_spectral(int32_t kind, int32_t nrow, int32_t k, int32_t ncol, int32_t ns, const void* A, const void* lam,
+                    const void* B, const double* s, void* out, int64_t so_t, int64_t so_a, int64_t so_b, double2* W,
+                    hipStream_t st) {
   const int64_t nw = (int64_t)ns * k;
-  hipLaunchKernelGGL(spectral_weights_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, (int)kind,
-                     (int)ns, (int)k, (const double2*)lam, s, W);
-  PQA_HIP(hipGetLastError());
   const bool wide = ncol >= 48;
   const int tbw = wide ? 4 * kTB : kTB;
   const dim3 grid((unsigned)((ns + kTT - 1) / kTT), (unsigned)((ncol + tbw - 1) / tbw), (unsigned)nrow);
   if (grid.y > 65535) return fail(PYQED_EINVAL, "spectral_sum: too many columns");
+  hipLaunchKernelGGL(spectral_weights_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, (int)kind,
+                     (int)ns, (int)k, (const double2*)lam, s, W);
+  PQA_HIP(hipGetLastError());
   const bool tfast = so_t == 1;
   const double2 *Wc = (const double2*)W, *Ac = (const double2*)A, *Bc = (const double2*)B;
   double2* O = (double2*)out;
@@ -246,4 +240,73 @@ extern "C" int pyqed_spectral_sum(int32_t kind, int32_t nrow, int32_t k, int32_t
                        (int)ncol, Wc, Ac, Bc, O, so_t, so_a, so_b);
   PQA_HIP(hipGetLastError());
   return PYQED_OK;
+}
+
+int check_sum_args(int32_t kind, int32_t nrow, int32_t k, int32_t ncol, int32_t ns, const void* A, const void* lam,
+                   const void* B, const double* s, void* out) {
+  if (kind != 0 && kind != 1) return fail(PYQED_EINVAL, "spectral_sum: kind must be 0 (exp) or 1 (resolvent)");
+  if (nrow < 1 || k < 1 || ncol < 1 || ns < 1)
+    return fail(PYQED_EINVAL, "spectral_sum: nrow, k, ncol and ns must be positive");
+  if (nrow > 65535) return fail(PYQED_EINVAL, "spectral_sum: at most 65535 rows per call");
+  if (!A || !lam || !B || !s || !out) return fail(PYQED_EINVAL, "spectral_sum: null pointer");
+  return PYQED_OK;
+}
+
+// Weight staging of pyqed_spectral_sum_static: PYQED_SPECTRAL_STATIC_MAX complex doubles (16 MiB) in
+// every GPU's copy of the library's code object, held until the process ends.  Host code never passes
+// `scratch` itself to a kernel (on the host that name is only a stand-in, not a device address): it
+// passes the address hipGetSymbolAddress gives for the current device, looked up once per device.
+__device__ __align__(256) double2 scratch[PYQED_SPECTRAL_STATIC_MAX];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double2* g_scratch_dev[kMaxDevices] = {};
+
+int scratch_of_current_device(double2** out) {
+  int dev = -1;
+  PQA_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(PYQED_EINVAL, "spectral_sum_static: device index " + std::to_string(dev) + " out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(PYQED_EHIP, std::string("spectral_sum_static: hipGetSymbolAddress(scratch): ") +
+                                  hipGetErrorString(e));
+    g_scratch_dev[dev] = (double2*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return PYQED_OK;
+}
+
+}  // namespace
+
+extern "C" int pyqed_spectral_sum(int32_t kind, int32_t nrow, int32_t k, int32_t ncol, int32_t ns, const void* A,
+                                  const void* lam, const void* B, const double* s, void* out, int64_t so_t,
+                                  int64_t so_a, int64_t so_b, void* work, int64_t work_bytes, void* stream) {
+  int rc = check_sum_args(kind, nrow, k, ncol, ns, A, lam, B, s, out);
+  if (rc) return rc;
+  if (!work) return fail(PYQED_EINVAL, "spectral_sum: null pointer");
+  if (work_bytes < pyqed_spectral_work_bytes(k, ns))
+    return fail(PYQED_EINVAL, "spectral_sum: workspace of " + std::to_string(work_bytes) + " bytes < " +
+                                  std::to_string(pyqed_spectral_work_bytes(k, ns)) + " needed");
+  return launch_spectral(kind, nrow, k, ncol, ns, A, lam, B, s, out, so_t, so_a, so_b, (double2*)work,
+                         (hipStream_t)stream);
+}
+
+extern "C" int pyqed_spectral_sum_static(int32_t kind, int32_t nrow, int32_t k, int32_t ncol, int32_t ns,
+                                         const void* A, const void* lam, const void* B, const double* s, void* out,
+                                         int64_t so_t, int64_t so_a, int64_t so_b, void* stream) {
+  int rc = check_sum_args(kind, nrow, k, ncol, ns, A, lam, B, s, out);
+  if (rc) return rc;
+  if ((int64_t)k * ns > (int64_t)PYQED_SPECTRAL_STATIC_MAX)
+    return fail(PYQED_EINVAL, "spectral_sum_static: k * ns = " + std::to_string((int64_t)k * ns) +
+                                  " weights exceed the static scratch of " +
+                                  std::to_string((int64_t)PYQED_SPECTRAL_STATIC_MAX) +
+                                  " (use pyqed_spectral_sum with a workspace)");
+  double2* W = nullptr;
+  if ((rc = scratch_of_current_device(&W))) return rc;
+  // every weight the GEMM reads (t < ns, j < k) is written by the first launch: nothing to clear
+  return launch_spectral(kind, nrow, k, ncol, ns, A, lam, B, s, out, so_t, so_a, so_b, W, (hipStream_t)stream);
 }
