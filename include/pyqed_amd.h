@@ -51,6 +51,13 @@ extern "C" {
                                     plan_layout's state offsets refer to that layout.  The stage inputs
                                     stay full matrices.  No counterpart in the reference (which stores
                                     full matrices); it halves the FP64 state traffic. */
+#define PYQED_PLAN_ADJOINT 32    /* the plan's operator is the adjoint L^H of the HEOM operator (Frobenius
+                                    inner product): same tables, H -> -H^H, Q_m -> Q_m^H, and the
+                                    coefficients of the minus and plus neighbour terms swapped and
+                                    conjugated.  pyqed_heom_rhs then applies L^H; one RK4 step of it with
+                                    the stage times reversed is the discrete adjoint of a forward step.
+                                    Generic kernels only; excludes PYQED_PLAN_HERMITIAN.  No counterpart in
+                                    the reference (which has no gradients). */
 
 /* Thread-local description of the last error (never NULL). */
 const char* pyqed_last_error(void);
@@ -181,6 +188,18 @@ int pyqed_heom_plan_class_state(const pyqed_heom_plan* plan, int64_t* state_elem
  * arrays [12]), then clears the record.  set_timing(0) stops recording. */
 int pyqed_heom_plan_set_timing(pyqed_heom_plan* plan, int32_t enable);
 int pyqed_heom_plan_read_timing(pyqed_heom_plan* plan, double* ms, int64_t* launches);
+
+/* Gradient contraction of the streamed HEOM adjoint (no counterpart in the reference): for every
+ * instance and field column c,
+ *   G[c] = sum_{s : col(s) = c} w[s] sum_a [x[s]_a, u[s]_a^H],   col(0..3) = (0, 1, 1, 2),
+ * over the instance's ADOs a.  x[s], u[s] are host arrays of 4 device pointers to full hierarchy
+ * matrices in pyqed_heom_rhs's layout (a NULL pair is skipped): x[s] the forward input of RK4 stage s,
+ * u[s] the adjoint variable of that stage's right-hand side divided by w[s].  G is device complex
+ * double [3][traj_stride], overwritten.  work: device, >= pyqed_heom_adjoint_work_bytes.  Used with
+ * a PYQED_PLAN_ADJOINT plan to differentiate DEOMSolver.run (pyqed/heom/deom.py:725-766, 1072-1114). */
+int pyqed_heom_adjoint_work_bytes(const pyqed_heom_plan* plan, int64_t* bytes);
+int pyqed_heom_adjoint_contract(pyqed_heom_plan* plan, const void* const* x, const void* const* u, const double* w,
+                                void* G, void* work, int64_t work_bytes, void* stream);
 
 /* ------------------------------------------------------------------ Liouville-space RK4
  * Batched propagation of vec(rho) (row-major, length N = d*d) under

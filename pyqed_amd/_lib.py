@@ -19,6 +19,7 @@ PLAN_DENSE_Q = 2
 PLAN_GENERIC = 4
 PLAN_HERMITIAN = 8
 PLAN_PACKED = 16
+PLAN_ADJOINT = 32
 
 SYMBOLS = [
     "pyqed_last_error", "pyqed_version", "pyqed_device_count",
@@ -26,6 +27,7 @@ SYMBOLS = [
     "pyqed_heom_tables_export", "pyqed_heom_tables_size", "pyqed_heom_tables_build", "pyqed_heom_run", "pyqed_heom_rhs", "pyqed_heom_plan_stats",
     "pyqed_heom_plan_classes", "pyqed_heom_plan_set_tiling", "pyqed_heom_plan_get_tiling", "pyqed_heom_plan_set_variant",
     "pyqed_heom_plan_class_state", "pyqed_heom_plan_set_timing", "pyqed_heom_plan_read_timing",
+    "pyqed_heom_adjoint_work_bytes", "pyqed_heom_adjoint_contract",
     "pyqed_liouville_run", "pyqed_liouville_adjoint", "pyqed_legacy_heom_run",
     "pyqed_spectral_work_bytes", "pyqed_spectral_sum", "pyqed_spectral_sum_static",
 ]
@@ -82,6 +84,8 @@ def lib():
     L.pyqed_heom_plan_class_state.argtypes = [vp, vp]
     L.pyqed_heom_plan_set_timing.argtypes = [vp, i32]
     L.pyqed_heom_plan_read_timing.argtypes = [vp, vp, vp]
+    L.pyqed_heom_adjoint_work_bytes.argtypes = [vp, C.POINTER(i64)]
+    L.pyqed_heom_adjoint_contract.argtypes = [vp, vp, vp, vp, vp, vp, i64, vp]
     L.pyqed_liouville_run.argtypes = [i32, i32, vp, i32, vp, vp, i64, dbl, i32, vp, vp, vp, i64, vp]
     L.pyqed_liouville_adjoint.argtypes = [i32, i32, vp, i32, vp, vp, i64, dbl, i32, vp, vp, vp, vp, vp, i64, vp]
     L.pyqed_legacy_heom_run.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, vp, i64, dbl, i32, vp, vp, vp]

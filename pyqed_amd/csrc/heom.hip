@@ -1563,6 +1563,7 @@ struct pyqed_heom_plan {
   double2* d_mat = nullptr;
   int32_t* d_res = nullptr;
   int32_t* d_site = nullptr;
+  int2* d_inst_items = nullptr;  // per instance, its work items [first, last) (the adjoint reduction)
   double2* d_coefp = nullptr;
   int n_proj = 0;
   std::vector<char> is_res;
@@ -1577,6 +1578,7 @@ struct pyqed_heom_plan {
   } pd[3];
   bool hermitian = false;
   bool packed = false;     // y, ynew hold upper triangles (PYQED_PLAN_PACKED)
+  bool adjoint = false;    // the plan's operator is the adjoint L^H (PYQED_PLAN_ADJOINT)
   int64_t n_state = 0;     // RK4 state elements (size of y): n_elems, or the packed count
   int64_t cls_state[3] = {0, 0, 0};  // RK4 state elements of the streamed items per class
   hipStream_t side = nullptr;
@@ -1632,6 +1634,26 @@ size_t resident_lds_bytes(const pyqed_heom_plan* P, int64_t n_el, size_t tsize) 
   return h + (size_t)n_el * (32 + 2 * 2 * tsize);
 }
 
+// The work queue and, per instance, the range [first, last) of its items: tile() lists the items of one
+// instance consecutively.
+int upload_items(pyqed_heom_plan* P) {
+  (void)hipFree(P->d_items);
+  (void)hipFree(P->d_inst_items);
+  P->d_items = nullptr;
+  P->d_inst_items = nullptr;
+  std::vector<int2> rng(P->n_inst, make_int2(0, 0));
+  std::vector<char> seen(P->n_inst, 0);
+  for (int it = 0; it < (int)P->items.size(); ++it) {
+    const int i = P->items[it].inst;
+    if (!seen[i]) rng[i].x = it;
+    seen[i] = 1;
+    rng[i].y = it + 1;
+  }
+  int rc = upload(P->items, &P->d_items);
+  if (rc) return rc;
+  return upload(rng, &P->d_inst_items);
+}
+
 int group_size(int d2) {
   int g = 4;
   while (g < d2 && g < 64) g <<= 1;
@@ -1649,6 +1671,7 @@ void destroy_plan(pyqed_heom_plan* P) {
   (void)hipFree(P->d_mat);
   (void)hipFree(P->d_res);
   (void)hipFree(P->d_site);
+  (void)hipFree(P->d_inst_items);
   (void)hipFree(P->d_coefp);
   if (P->side) (void)hipStreamDestroy(P->side);
   if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
@@ -1719,10 +1742,15 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
   P->flags = flags;
   P->hermitian = (flags & PYQED_PLAN_HERMITIAN) != 0;
   P->packed = (flags & PYQED_PLAN_PACKED) != 0;
+  P->adjoint = (flags & PYQED_PLAN_ADJOINT) != 0;
   const bool dense = (flags & PYQED_PLAN_DENSE_Q) != 0;
   if (P->packed && !P->hermitian) {
     destroy_plan(P);
     return fail(PYQED_EINVAL, "plan_create: PYQED_PLAN_PACKED needs PYQED_PLAN_HERMITIAN");
+  }
+  if (P->adjoint && P->hermitian) {
+    destroy_plan(P);
+    return fail(PYQED_EINVAL, "plan_create: PYQED_PLAN_ADJOINT excludes PYQED_PLAN_HERMITIAN and PYQED_PLAN_PACKED");
   }
   // ---- validate and build tables
   for (int i = 0; i < n_inst; ++i) {
@@ -1804,22 +1832,45 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     for (int k = 0; k < s.K; ++k) {
       const std::complex<double> I1(0.0, 1.0);
       const std::complex<double> sq = std::sqrt(cz(s.etaa, k));
-      const std::complex<double> v[5] = {cz(s.expn, k), -I1 * cz(s.etal, k) / sq, I1 * cz(s.etar, k) / sq, -I1 * sq,
-                                         I1 * sq};
+      std::complex<double> v[5] = {cz(s.expn, k), -I1 * cz(s.etal, k) / sq, I1 * cz(s.etar, k) / sq, -I1 * sq,
+                                   I1 * sq};
+      if (P->adjoint) {
+        // block (n, n - e_k) of L^H is the conjugate of block (n - e_k, n) of L, a plus term, and
+        // block (n, n + e_k) that of the minus term of n + e_k: swap the two pairs and conjugate
+        const std::complex<double> a[5] = {std::conj(v[0]), std::conj(v[3]), std::conj(v[4]), std::conj(v[1]),
+                                           std::conj(v[2])};
+        for (int q = 0; q < 5; ++q) v[q] = a[q];
+      }
       for (int q = 0; q < 5; ++q) coef.push_back(make_double2(v[q].real(), v[q].imag()));
       mode.push_back(s.mode[k]);
     }
+    // the adjoint operator: -i[H, .]^H = -i[-H^H, .] and (A . B)^H = A^H . B^H for the couplings
+    auto push_mat = [&](const double* src, int dd, double sign, bool adj) {
+      for (int e = 0; e < dd * dd; ++e) {
+        const int a2 = e / dd, b2 = e - a2 * dd;
+        const int f = adj ? b2 * dd + a2 : e;
+        mat.push_back(make_double2(sign * src[2 * f], (adj ? -sign : sign) * src[2 * f + 1]));
+      }
+    };
+    auto push_diag = [&](const double* src, int n, bool adj) {
+      for (int e = 0; e < n; ++e) mat.push_back(make_double2(src[2 * e], adj ? -src[2 * e + 1] : src[2 * e + 1]));
+    };
+    const bool adj = P->adjoint;
     I.h_off = (int)mat.size();
-    for (int e = 0; e < I.d2; ++e) mat.push_back(make_double2(s.H[2 * e], s.H[2 * e + 1]));
-    if (s.sdip)
-      for (int e = 0; e < I.d2; ++e) mat.push_back(make_double2(s.sdip[2 * e], s.sdip[2 * e + 1]));
+    push_mat(s.H, s.d, adj ? -1.0 : 1.0, adj);
+    if (s.sdip) push_mat(s.sdip, s.d, adj ? -1.0 : 1.0, adj);
     I.q_off = (int)mat.size();
-    const int nq = dense ? s.ncoup * I.d2 : s.ncoup * s.d;
     const double* qsrc = dense ? s.qmat : s.qdiag;
     const double* csrc = dense ? s.cmat : s.cdiag;
-    for (int e = 0; e < nq; ++e) mat.push_back(make_double2(qsrc[2 * e], qsrc[2 * e + 1]));
+    for (int m = 0; m < s.ncoup; ++m) {
+      if (dense) push_mat(qsrc + 2 * (int64_t)m * I.d2, s.d, 1.0, adj);
+      else push_diag(qsrc + 2 * (int64_t)m * s.d, s.d, adj);
+    }
     if (csrc)
-      for (int e = 0; e < nq; ++e) mat.push_back(make_double2(csrc[2 * e], csrc[2 * e + 1]));
+      for (int m = 0; m < s.ncoup; ++m) {
+        if (dense) push_mat(csrc + 2 * (int64_t)m * I.d2, s.d, 1.0, adj);
+        else push_diag(csrc + 2 * (int64_t)m * s.d, s.d, adj);
+      }
     if (flags & PYQED_PLAN_HERMITIAN) {
       // the caller asserts Hermitian initial states; check what makes the dynamics preserve it
       bool ok = !dense && !s.cdiag;
@@ -1842,7 +1893,7 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     }
     // ---- site-projector path: every coupling is c_m |p_m><p_m| (diagonal with one nonzero entry)
     I.path = 0;
-    if (!dense && !s.cdiag && s.d >= 6 && !(flags & PYQED_PLAN_GENERIC)) {
+    if (!dense && !s.cdiag && s.d >= 6 && !(flags & PYQED_PLAN_GENERIC) && !P->adjoint) {
       std::vector<int> psite(s.ncoup, -1);
       std::vector<std::complex<double>> pw(s.ncoup);
       bool proj = true;
@@ -1946,7 +1997,7 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
   }
   // ---- upload
   int rc = 0;
-  if ((rc = upload(P->inst, &P->d_inst)) || (rc = upload(P->items, &P->d_items)) ||
+  if ((rc = upload(P->inst, &P->d_inst)) || (rc = upload_items(P)) ||
       (rc = upload(rowptr, &P->d_rowptr)) || (rc = upload(ent, &P->d_ent)) || (rc = upload(coef, &P->d_coef)) ||
       (rc = upload(mode, &P->d_mode)) || (rc = upload(mat, &P->d_mat)) || (rc = upload(P->resident, &P->d_res)) ||
       (rc = upload(sites, &P->d_site)) ||
@@ -2035,9 +2086,7 @@ extern "C" int pyqed_heom_plan_set_tiling(pyqed_heom_plan* P, int64_t target_ele
   if (!P || blocks_per_cu < 1 || target_elems < 0) return fail(PYQED_EINVAL, "plan_set_tiling: bad argument");
   PQA_HIP(hipDeviceSynchronize());  // the old queue may still be in use by queued launches
   retile(P, target_elems, blocks_per_cu);
-  (void)hipFree(P->d_items);
-  P->d_items = nullptr;
-  return upload(P->items, &P->d_items);
+  return upload_items(P);
 }
 
 extern "C" int pyqed_heom_plan_set_variant(pyqed_heom_plan* P, int32_t variant) {
@@ -2377,6 +2426,165 @@ extern "C" int pyqed_heom_run(pyqed_heom_plan* P, int32_t precision, void* y, vo
     return run_typed<double>(P, (double2*)y, (char*)workspace, field, step0, nsteps, dt, (double2*)traj, save_every,
                              st);
   return run_typed<float>(P, (double2*)y, (char*)workspace, field, step0, nsteps, dt, (double2*)traj, save_every, st);
+}
+
+namespace {
+
+// ------------------------------------------------------------------ adjoint contraction
+// For the gradients of the discrete adjoint: per instance and field column c,
+//   G[c] = sum_{s : col(s) = c} w_s sum_a [X_s,a, U_s,a^H],   col = (0, 1, 1, 2) for s = 0..3,
+// where X_s is the forward input of RK4 stage s, U_s the matching adjoint stage input and a runs over
+// the instance's ADOs.  Then dJ/dH (as a cotangent) is i sum G^H and dJ/df = Re(-i Tr(mu G)) for a
+// field f entering H(t) = H + mu f.
+struct AdjParams {
+  const InstDev* insts;
+  const WorkItem* items;
+  const int2* inst_items;
+  const double2* x[4];
+  const double2* u[4];
+  double w[4];
+  double2* part;  // [nitems][3][d2max]
+  double2* G;     // [3][traj_stride]
+  int nitems, d2max;
+  int64_t traj_stride;
+};
+
+constexpr int kAdjMaxEl = 4;  // elements per lane: I.grp >= d2 lanes per ADO for d2 <= 64, else 256 lanes
+
+__global__ __launch_bounds__(kThreads) void heom_adjoint_contract_kernel(AdjParams q) {
+  extern __shared__ __align__(16) char smem[];
+  const int tid = threadIdx.x;
+  for (int it = blockIdx.x; it < q.nitems; it += gridDim.x) {
+    const WorkItem wi = q.items[it];
+    const InstDev I = q.insts[wi.inst];
+    const int d = I.d, d2 = I.d2, G = d2 <= 64 ? I.grp : kThreads;
+    const int nslot = kThreads / G, slot = tid / G, le = tid - slot * G;
+    double2* sx = (double2*)smem + (int64_t)slot * 2 * d2;
+    double2* su = sx + d2;
+    cpx<double> acc[3][kAdjMaxEl];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int t = 0; t < kAdjMaxEl; ++t) acc[c][t] = mk(0.0, 0.0);
+    for (int a0 = wi.ado0; a0 < wi.ado0 + wi.nado; a0 += nslot) {
+      const int a = a0 + slot;
+      const bool act = a < wi.ado0 + wi.nado;
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+        if (!q.x[st] || !q.u[st]) continue;  // block-uniform
+        const int c = st == 0 ? 0 : (st == 3 ? 2 : 1);
+        __syncthreads();
+        if (act) {
+          const int64_t base = I.off + (int64_t)a * d2;
+          for (int e = le; e < d2; e += G) {
+            sx[e] = q.x[st][base + e];
+            su[e] = q.u[st][base + e];
+          }
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+          for (int t = 0; t < kAdjMaxEl; ++t) {
+            const int e = le + t * G;
+            if (e >= d2) break;
+            const int i = e / d, j = e - i * d;
+            cpx<double> r = mk(0.0, 0.0);
+            for (int k = 0; k < d; ++k) {
+              const double2 xa = sx[i * d + k], ua = su[j * d + k];  // x_ik conj(u_jk)
+              const double2 ub = su[k * d + i], xb = sx[k * d + j];  // conj(u_ki) x_kj
+              r.x += xa.x * ua.x + xa.y * ua.y - (ub.x * xb.x + ub.y * xb.y);
+              r.y += xa.y * ua.x - xa.x * ua.y - (ub.x * xb.y - ub.y * xb.x);
+            }
+            acc[c][t] = acc[c][t] + r * q.w[st];
+          }
+        }
+      }
+    }
+    // sum over the slots of the block, then one partial per item
+    __syncthreads();
+    double2* red = (double2*)smem;  // [nslot][3][d2]
+#pragma unroll
+    for (int t = 0; t < kAdjMaxEl; ++t) {
+      const int e = le + t * G;
+      if (e >= d2) break;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) red[((int64_t)slot * 3 + c) * d2 + e] = make_double2(acc[c][t].x, acc[c][t].y);
+    }
+    __syncthreads();
+    for (int e = tid; e < 3 * d2; e += kThreads) {
+      double2 sum = make_double2(0.0, 0.0);
+      for (int sl = 0; sl < nslot; ++sl) {
+        const double2 v = red[(int64_t)sl * 3 * d2 + e];
+        sum.x += v.x;
+        sum.y += v.y;
+      }
+      const int c = e / d2, f = e - c * d2;
+      q.part[((int64_t)it * 3 + c) * q.d2max + f] = sum;
+    }
+  }
+}
+
+// One block per instance: G[c][traj_off + f] = sum over the instance's items of their partials.
+__global__ __launch_bounds__(kThreads) void heom_adjoint_reduce_kernel(AdjParams q) {
+  const int i = blockIdx.x;
+  const InstDev I = q.insts[i];
+  const int2 r = q.inst_items[i];
+  for (int e = threadIdx.x; e < 3 * I.d2; e += kThreads) {
+    const int c = e / I.d2, f = e - c * I.d2;
+    double2 sum = make_double2(0.0, 0.0);
+    for (int it = r.x; it < r.y; ++it) {
+      const double2 v = q.part[((int64_t)it * 3 + c) * q.d2max + f];
+      sum.x += v.x;
+      sum.y += v.y;
+    }
+    q.G[c * q.traj_stride + I.traj_off + f] = sum;
+  }
+}
+
+size_t adjoint_lds_bytes(const pyqed_heom_plan* P) {
+  // staging: nslot * 2 * d2 <= 2 * max(256, d2max) complex; reduction: nslot * 3 * d2 <= 3 * max(256, d2max)
+  return (size_t)3 * std::max(256, P->d2max) * sizeof(double2);
+}
+
+}  // namespace
+
+extern "C" int pyqed_heom_adjoint_work_bytes(const pyqed_heom_plan* P, int64_t* bytes) {
+  if (!P || !bytes) return fail(PYQED_EINVAL, "heom_adjoint_work_bytes: null argument");
+  *bytes = (int64_t)P->items.size() * 3 * P->d2max * (int64_t)sizeof(double2);
+  return PYQED_OK;
+}
+
+extern "C" int pyqed_heom_adjoint_contract(pyqed_heom_plan* P, const void* const* x, const void* const* u,
+                                           const double* w, void* G, void* work, int64_t work_bytes, void* stream) {
+  if (!P || !x || !u || !w || !G || !work) return fail(PYQED_EINVAL, "heom_adjoint_contract: null argument");
+  int64_t need = 0;
+  pyqed_heom_adjoint_work_bytes(P, &need);
+  if (work_bytes < need)
+    return fail(PYQED_EINVAL, "heom_adjoint_contract: workspace of " + std::to_string(work_bytes) + " bytes < " +
+                                  std::to_string(need) + " needed");
+  if (P->items.empty()) return PYQED_OK;
+  AdjParams q;
+  std::memset(&q, 0, sizeof(q));
+  q.insts = P->d_inst;
+  q.items = P->d_items;
+  q.inst_items = P->d_inst_items;
+  for (int s = 0; s < 4; ++s) {
+    q.x[s] = (const double2*)x[s];
+    q.u[s] = (const double2*)u[s];
+    q.w[s] = w[s];
+  }
+  q.part = (double2*)work;
+  q.G = (double2*)G;
+  q.nitems = (int)P->items.size();
+  q.d2max = P->d2max;
+  q.traj_stride = P->traj_stride;
+  const hipStream_t st = (hipStream_t)stream;
+  const int grid = std::max(1, std::min(q.nitems, P->num_cu * 8));
+  hipLaunchKernelGGL(heom_adjoint_contract_kernel, dim3(grid), dim3(kThreads), adjoint_lds_bytes(P), st, q);
+  PQA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(heom_adjoint_reduce_kernel, dim3(P->n_inst), dim3(kThreads), 0, st, q);
+  PQA_HIP(hipGetLastError());
+  return PYQED_OK;
 }
 
 extern "C" int pyqed_heom_rhs(pyqed_heom_plan* P, const void* x, void* out, const double* field, int64_t step,

@@ -112,7 +112,7 @@ def unpack_upper(v, d):
 class HeomEngine:
     """A plan over one or more HEOM instances plus its device state."""
 
-    def __init__(self, instances, *, resident=True, generic=False, hermitian=None, packed=None):
+    def __init__(self, instances, *, resident=True, generic=False, hermitian=None, packed=None, adjoint=False):
         self.torch = torch = _lib.require_gpu()
         self.L = _lib.lib()
         self.prep = [prepare_instance(i) for i in instances]
@@ -139,13 +139,15 @@ class HeomEngine:
             for name, b in bufs.items():
                 setattr(s, name, None if b is None else b.ctypes.data)
         if hermitian is None:
-            hermitian = not self.dense and all(_hermitian_dynamics(p) for p in self.prep)
+            hermitian = not adjoint and not self.dense and all(_hermitian_dynamics(p) for p in self.prep)
         self.hermitian = bool(hermitian)
         # Hermitian hierarchies keep only the upper triangle of every ADO in the FP64 state
         self.packed = self.hermitian if packed is None else bool(packed)
+        # adjoint: the plan applies L^H (same layout and tables), for the streamed HEOM adjoint
+        self.adjoint = bool(adjoint)
         flags = ((0 if resident else _lib.PLAN_NO_RESIDENT) | (_lib.PLAN_DENSE_Q if self.dense else 0) |
                  (_lib.PLAN_GENERIC if generic else 0) | (_lib.PLAN_HERMITIAN if self.hermitian else 0) |
-                 (_lib.PLAN_PACKED if self.packed else 0))
+                 (_lib.PLAN_PACKED if self.packed else 0) | (_lib.PLAN_ADJOINT if self.adjoint else 0))
         plan = C.c_void_p()
         _lib.check(self.L.pyqed_heom_plan_create(arr, self.n, flags, C.byref(plan)))
         self.plan = plan
