@@ -132,33 +132,118 @@ def fp64_plain(insts, dt, warmup, steps, torch):
     return ados * steps / total, total, finals
 
 
-def cpu_baseline(insts, dt, budget_s=15.0):
-    """Oracle (numpy FP64) on a stratified sample: one instance per (d, depth) stratum, smallest first,
-    one RK4 step each, until the time budget is spent."""
-    from threadpoolctl import threadpool_limits
-    from oracle import heom as O
+def cpu_info():
+    """Host CPU model and the cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def _strata(insts):
+    """One instance per (d, depth) stratum: {(d, L): index}."""
     strata = {}
     for i, p in enumerate(insts):
-        key = (p["H"].shape[0], p["L"])
-        strata.setdefault(key, i)
-    order = sorted(strata.items(), key=lambda kv: (kv[0][0], kv[0][1]))
-    done, ados, t_used = [], 0, 0.0
-    with threadpool_limits(limits=1):
-        for (d, L), i in order:
+        strata.setdefault((p["H"].shape[0], p["L"]), i)
+    return strata
+
+
+class _ChunkedRHS:
+    """The vectorised oracle right-hand side (diagonal couplings as row/column scalings) split over ADO
+    ranges, evaluated by a thread pool: numpy releases the GIL inside these array kernels."""
+
+    def __init__(self, H, qd, bath, mode, L, nthreads):
+        from oracle import heom as O
+        self.O = O
+        hier = O.Hierarchy(len(mode), L)
+        self.nmax = hier.nmax
+        self.H = np.asarray(H, np.complex128)
+        terms = O.DiagTerms(hier, bath, mode, qd)
+        self.damp = terms.damp
+        bounds = np.linspace(0, hier.nmax, min(nthreads * 4, hier.nmax) + 1).astype(int)
+        self.chunks = []
+        for a0, a1 in zip(bounds[:-1], bounds[1:]):
+            if a1 <= a0:
+                continue
+            tl = []
+            for sel, nb, c, cm in terms.terms:
+                m = (sel >= a0) & (sel < a1)
+                if m.any():
+                    tl.append((sel[m] - a0, nb[m], c[m], cm))
+            self.chunks.append((a0, a1, tl))
+
+    def __call__(self, rho, pool):
+        out = np.empty_like(rho)
+        H = self.H
+
+        def work(ch):
+            a0, a1, tl = ch
+            x = rho[a0:a1]
+            o = -self.damp[a0:a1] * x - 1j * (np.matmul(H[None], x) - np.matmul(x, H[None]))
+            for sel, nb, c, cm in tl:
+                o[sel] += c * cm[None] * rho[nb]
+            out[a0:a1] = o
+        list(pool.map(work, self.chunks))
+        return out
+
+
+def cpu_baseline(insts, dt, r0_budget_s=10.0, workers=None):
+    """CPU baselines on the box's host cores, on a stratified sample of the workload (one instance per
+    (d, depth) stratum, or the one instance):
+      R1  all cores: every sampled instance one RK4 step with the vectorised numpy FP64 oracle, each
+          stage split over ADO ranges on min(16, usable cores) threads; the headline cpu_baseline value
+          (ADO-updates of the sample / time);
+      R0  one core: the per-ADO loop with the reference's structure (oracle rhs_loop, deom.py:641-673),
+          one RK4 step per stratum, smallest first, until r0_budget_s is spent."""
+    from concurrent.futures import ThreadPoolExecutor
+    from math import comb
+    from threadpoolctl import threadpool_limits
+    from oracle import heom as O
+    model, host_cores = cpu_info()
+    strata = _strata(insts)
+    nthr = max(1, min(workers or 16, host_cores))
+    ados, wall = 0, 0.0
+    with threadpool_limits(limits=1), ThreadPoolExecutor(nthr) as pool:
+        for (d, L), i in sorted(strata.items()):
             p = insts[i]
             etal, etar, etaa, expn, mode = p["bath"]
+            qd = np.array([np.diag(q) for q in np.asarray(p["Q"])])
+            f = _ChunkedRHS(p["H"], qd, (etal, etar, etaa, expn), np.asarray(mode), L, nthr)
+            y = np.zeros((f.nmax, d, d), np.complex128)
+            y[0] = p["rho0"]
             t0 = time.perf_counter()
-            O.run(p["H"], list(p["Q"]), (etal, etar, etaa, expn), mode, L, p["rho0"], dt, 1)
-            t_used += time.perf_counter() - t0
-            from math import comb
-            ados += comb(len(mode) + L, L)
+            O.rk4_step(y, lambda x, t: f(x, pool), dt, 0.0)
+            wall += time.perf_counter() - t0
+            ados += f.nmax
+    r1 = dict(value=ados / wall, unit="ADO-updates/s", cores=nthr, seconds=wall, ado_updates=ados,
+              strata=len(strata))
+    # R0: the reference's per-ADO loop structure on one core
+    done, ados0, used = [], 0, 0.0
+    with threadpool_limits(limits=1):
+        for (d, L), i in sorted(strata.items()):
+            p = insts[i]
+            etal, etar, etaa, expn, mode = p["bath"]
+            t1 = time.perf_counter()
+            O.run(p["H"], list(p["Q"]), (etal, etar, etaa, expn), mode, L, p["rho0"], dt, 1, vectorised=False)
+            used += time.perf_counter() - t1
+            ados0 += comb(len(mode) + L, L)
             done.append(f"d={d},L={L}")
-            if t_used > budget_s:
+            if used > r0_budget_s:
                 break
-    return dict(value=ados / t_used, unit="ADO-updates/s", cores=1, kind="port",
-                sample="oracle.heom.run (numpy FP64, vectorised over ADOs, 1 thread), 1 RK4 step of one "
-                       f"instance per (d, depth) stratum: {len(done)} strata ({done[0]} .. {done[-1]}), "
-                       f"{ados} ADO-updates in {t_used:.1f} s")
+    r0 = dict(value=ados0 / used, unit="ADO-updates/s", cores=1, seconds=used, ado_updates=ados0,
+              strata=len(done), sample=f"{done[0]} .. {done[-1]}")
+    return dict(value=r1["value"], unit="ADO-updates/s", cores=nthr, kind="port",
+                sample=(f"R1: one RK4 step of one instance per (d, depth) stratum ({len(strata)} strata, {ados} "
+                        f"ADO-updates), numpy FP64 oracle vectorised over ADOs with the diagonal couplings as "
+                        f"row/column scalings, each stage split over ADO ranges on {nthr} threads, {wall:.1f} s"),
+                cpu_model=model, host_cores=host_cores, R1=r1,
+                R0=dict(r0, note="per-ADO loop with the reference's structure (oracle rhs_loop), one thread"))
 
 
 def kernel_name(stage, cls, precision, eng):
@@ -168,9 +253,10 @@ def kernel_name(stage, cls, precision, eng):
     pack = "true" if eng.packed else "false"
     if cls == 0:
         return f"heom_stage_kernel<{T}, {TI}, {T}, {stage}, {pack}>"
-    mfma = "true" if (cls == 2 and precision == "mixed") else "false"
+    mfma = "true" if (cls >= 2 and precision == "mixed") else "false"
     herm = "true" if eng.hermitian else "false"
-    return f"heom_proj_kernel<{T}, {TI}, {T}, {stage}, {mfma}, {herm}, {pack}>"
+    nt = 2 if (cls == 3 and precision == "mixed") else 1
+    return f"heom_proj_kernel<{T}, {TI}, {T}, {stage}, {mfma}, {herm}, {pack}, {nt}>"
 
 
 def measured_traffic(name, workload, packed):
@@ -730,8 +816,9 @@ def main():
     ts = 8 if args.precision == "mixed" else 16
     F = eng.classes()["elems"]
     S = eng.class_state()
-    step_bytes = sum(alg_bytes(s, ts, F[c]) for s in range(4) for c in range(3))
-    step_layout = sum(stage_bytes(s, ts, F[c], S[c], eng.packed) for s in range(4) for c in range(3))
+    nc = len(F)
+    step_bytes = sum(alg_bytes(s, ts, F[c]) for s in range(4) for c in range(nc))
+    step_layout = sum(stage_bytes(s, ts, F[c], S[c], eng.packed) for s in range(4) for c in range(nc))
     t_step = float(np.mean(step_ms)) / 1e3
     roofline = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
                 "bytes_model": "classic three-register RK4, %d B per full-matrix element per step"
@@ -742,10 +829,10 @@ def main():
     if timing:
         ms, launches = eng.read_timing()
         eng.set_timing(False)
-        avg = {k: ms[k] / launches[k] for k in range(12) if launches[k]}
+        avg = {k: ms[k] / launches[k] for k in range(len(ms)) if launches[k]}
     if avg:  # no streamed launches when every instance runs LDS-resident
         k = max(avg, key=avg.get)
-        s_dom, c_dom = divmod(k, 3)
+        s_dom, c_dom = divmod(k, nc)
         b = alg_bytes(s_dom, ts, F[c_dom])
         bl = stage_bytes(s_dom, ts, F[c_dom], S[c_dom], eng.packed)
         name = kernel_name(s_dom, c_dom, args.precision, eng)
@@ -753,7 +840,7 @@ def main():
                          "bytes_per_launch": b, "achieved": b / (avg[k] / 1e3) / 1e9,
                          "frac": b / (avg[k] / 1e3) / 1e9 / HBM_PEAK_GBS,
                          "layout_bytes_per_launch": bl, "layout_achieved": bl / (avg[k] / 1e3) / 1e9,
-                         "launch_ms_all": {f"s{kk // 3}c{kk % 3}": v for kk, v in sorted(avg.items())}})
+                         "launch_ms_all": {f"s{kk // nc}c{kk % nc}": v for kk, v in sorted(avg.items())}})
         roofline["traffic"] = measured_traffic(name, desc, eng.packed)
     else:
         roofline.update({"kernel": "all stage kernels of a step" if eng.stats()["stream_items"] else

@@ -173,10 +173,11 @@ int pyqed_heom_plan_get_tiling(const pyqed_heom_plan* plan, int64_t* target_elem
  * autotuner; results are identical. */
 int pyqed_heom_plan_set_variant(pyqed_heom_plan* plan, int32_t variant);
 /* Streamed work per kernel class (0 generic, 1 site-projector with VALU
- * commutator, 2 site-projector with MFMA commutator): items and elements
- * (host arrays [3], either may be NULL). */
+ * commutator, 2 site-projector with MFMA commutator on one 16 x 16 tile (d <= 16),
+ * 3 the same on 2 x 2 tiles (17 <= d <= 32)): items and elements
+ * (host arrays [4], either may be NULL). */
 int pyqed_heom_plan_classes(const pyqed_heom_plan* plan, int64_t* items, int64_t* elems);
-/* RK4-state elements (size in y) of the streamed work per kernel class (host array [3]): equal to
+/* RK4-state elements (size in y) of the streamed work per kernel class (host array [4]): equal to
  * the element counts of plan_classes, or the packed counts with PYQED_PLAN_PACKED.  No counterpart
  * in the reference; used for the benchmark's byte model. */
 int pyqed_heom_plan_class_state(const pyqed_heom_plan* plan, int64_t* state_elems);
@@ -184,8 +185,8 @@ int pyqed_heom_plan_class_state(const pyqed_heom_plan* plan, int64_t* state_elem
  * benchmark's roofline; no counterpart in the reference).  set_timing(1) records an event pair
  * around every kernel-class launch of later pyqed_heom_run calls (launches captured into a graph
  * are not timed); read_timing waits for the recorded launches and returns, for RK4 stage s (0..3)
- * and class c (0..2) at index 3*s + c, the summed milliseconds and the number of launches (host
- * arrays [12]), then clears the record.  set_timing(0) stops recording. */
+ * and class c (0..3) at index 4*s + c, the summed milliseconds and the number of launches (host
+ * arrays [16]), then clears the record.  set_timing(0) stops recording. */
 int pyqed_heom_plan_set_timing(pyqed_heom_plan* plan, int32_t enable);
 int pyqed_heom_plan_read_timing(pyqed_heom_plan* plan, double* ms, int64_t* launches);
 

@@ -169,13 +169,53 @@ def multi_drude_bath(lams, gams, betas, npsd):
 
 
 # ----------------------------------------------------------------------------- right-hand side
+def ranks(keys, comb):
+    """ado_rank of every row of keys [N, K] at once: sum_i C(s_i + i, i + 1) over prefix sums s_i."""
+    keys = np.asarray(keys, np.int64)
+    K = keys.shape[1]
+    s = np.cumsum(keys, axis=1)
+    i = np.arange(K)
+    return comb[s + i[None, :], (i + 1)[None, :]].sum(axis=1)
+
+
+def tables_vectorised(K, L):
+    """keys, plus and minus as gen_keys / neighbour_tables give them, built with array operations:
+    every multi-index of total <= L is placed at its rank, and the neighbours are the ranks of
+    keys +- e_k."""
+    comb = comb_table(K, L)
+    nmax = int(comb[L + K, L])
+    rows = [np.zeros((1, 0), np.int64)]
+    for k in range(K):  # all prefixes of length k + 1 with total <= L
+        prev = rows[-1]
+        tot = prev.sum(1) if prev.shape[1] else np.zeros(prev.shape[0], np.int64)
+        reps = L - tot + 1
+        new = np.repeat(prev, reps, axis=0)
+        val = np.concatenate([np.arange(r) for r in reps])
+        rows.append(np.concatenate([new, val[:, None]], axis=1))
+    allk = rows[-1] if K else np.zeros((1, 0), np.int64)
+    keys = np.zeros((nmax, K), np.int64)
+    if K:
+        keys[ranks(allk, comb)] = allk
+    plus = -np.ones((nmax, K), np.int64)
+    minus = -np.ones((nmax, K), np.int64)
+    tier = keys.sum(1)
+    for k in range(K):
+        e = np.zeros(K, np.int64)
+        e[k] = 1
+        up = tier < L
+        plus[up, k] = ranks(keys[up] + e, comb)
+        dn = keys[:, k] > 0
+        minus[dn, k] = ranks(keys[dn] - e, comb)
+    return keys, plus, minus
+
+
 class Hierarchy:
-    """Tables and coefficients for one HEOM instance."""
+    """Tables and coefficients for one HEOM instance (tables from tables_vectorised, which equal the
+    loop restatements gen_keys / neighbour_tables)."""
 
     def __init__(self, K, L):
         self.K, self.L = K, L
-        self.keys = gen_keys(K, L)
-        self.plus, self.minus = neighbour_tables(self.keys, L)
+        self.keys, self.plus, self.minus = tables_vectorised(K, L)
         self.nmax = self.keys.shape[0]
         self.tier = self.keys.sum(1)
 
@@ -225,6 +265,39 @@ def rhs_vec(rho, H, Q, hier, bath, mode):
             n = keys[sel, k].astype(np.float64)
             c = (-1j * np.sqrt(n + 1) * sq_eta[k])[:, None, None]
             out[sel] += c * (np.matmul(q[None], p) - np.matmul(p, q[None]))
+    return out
+
+
+class DiagTerms:
+    """Per-exponential gathers and coefficient matrices for rhs_vec_diag (diagonal couplings)."""
+
+    def __init__(self, hier, bath, mode, qdiag):
+        etal, etar, etaa, expn = bath
+        q = np.asarray(qdiag)
+        self.terms = []
+        sq = np.sqrt(etaa)
+        for k in range(hier.K):
+            qk = q[mode[k]]
+            sel = np.nonzero(hier.minus[:, k] >= 0)[0]
+            if sel.size:
+                n = hier.keys[sel, k].astype(np.float64)
+                cm = etal[k] * qk[:, None] - etar[k] * qk[None, :]  # (Q X)_ij = q_i X_ij, (X Q)_ij = X_ij q_j
+                self.terms.append((sel, hier.minus[sel, k], (-1j * np.sqrt(n) / sq[k])[:, None, None], cm))
+            sel = np.nonzero(hier.plus[:, k] >= 0)[0]
+            if sel.size:
+                n = hier.keys[sel, k].astype(np.float64)
+                cp = qk[:, None] - qk[None, :]
+                self.terms.append((sel, hier.plus[sel, k], (-1j * np.sqrt(n + 1) * sq[k])[:, None, None], cp))
+        self.damp = (hier.keys * np.asarray(expn)[None, :]).sum(1)[:, None, None]
+
+
+def rhs_vec_diag(rho, H, terms):
+    """rhs_vec for diagonal couplings Q_m = diag(q_m): every coupling product is a row or column scaling,
+    so each neighbour term is one gather and one elementwise multiply-add.  The all-core CPU baseline
+    uses it (same operator as rhs_vec; summation order differs at rounding level)."""
+    out = -terms.damp * rho - 1j * (np.matmul(H[None], rho) - np.matmul(rho, H[None]))
+    for sel, nb, c, cm in terms.terms:
+        out[sel] += c * cm[None] * rho[nb]
     return out
 
 

@@ -57,6 +57,9 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kMaxD = 32;
+// Streamed work classes: 0 generic, 1 site projector with VALU commutator, 2 site projector with MFMA
+// commutator on one 16 x 16 tile (d <= 16), 3 the same on 2 x 2 tiles (17 <= d <= 32)
+constexpr int kClasses = 4;
 constexpr int kMaxK = 255;
 constexpr int kMaxL = 126;
 
@@ -70,7 +73,7 @@ struct InstDev {
   int32_t rowptr_off, ent_off, coef_off, mode_off;
   int32_t h_off, q_off;  // offsets (complex elements) into the matrix pool
   // site-projector path: site-ordered tables and coefficients scaled by the projector weight
-  int32_t path;          // 0 generic, 1 projector/VALU commutator, 2 projector/MFMA commutator
+  int32_t path;          // work class: 0 generic, 1 projector/VALU, 2 projector/MFMA 16, 3 projector/MFMA 2x2x16
   int32_t coefp_off;     // coefficients scaled by the projector weights
   int32_t site_off;      // site of every exponential, [K]
   int32_t dp;            // RK4 state elements per ADO: d*d, or d(d+1)/2 in packed Hermitian mode
@@ -599,6 +602,12 @@ constexpr int kMaxEnt = 64;  // neighbour entries per ADO handled by the project
 #define PQA_CHUNK 2
 #endif
 constexpr int kChunk = PQA_CHUNK;  // neighbour loads per lane issued together
+#ifndef PQA_CHUNK_WIDE
+#define PQA_CHUNK_WIDE 8  // the same for the wide MFMA class (17 <= d <= 32: about 50 neighbour rows of d)
+#endif
+#ifndef PQA_WIDE_WAVES
+#define PQA_WIDE_WAVES 2  // minimum waves per SIMD of the wide MFMA class (LDS allows two blocks per CU)
+#endif
 #ifndef PQA_CHUNK_S1
 #define PQA_CHUNK_S1 3  // the same for stage 1 of the MFMA projector class
 #endif
@@ -717,16 +726,24 @@ __device__ __forceinline__ void proj_header(const StageParams& p, const InstDev&
 }
 
 // Minimum waves per SIMD each form of the projector kernel is compiled for.
-constexpr int proj_waves(bool mfma, bool f32, bool pack, int stage) {
-  return mfma ? (pack ? (stage == 3 ? PQA_PACK_WAVES_LAST : PQA_PACK_WAVES) : PQA_PROJ_WAVES)
-              : (f32 ? PQA_VALU_WAVES : PQA_PROJ_WAVES);
+constexpr int proj_waves(bool mfma, bool f32, bool pack, int stage, int nt) {
+  return (mfma && nt == 2) ? PQA_WIDE_WAVES
+         : mfma ? (pack ? (stage == 3 ? PQA_PACK_WAVES_LAST : PQA_PACK_WAVES) : PQA_PROJ_WAVES)
+                : (f32 ? PQA_VALU_WAVES : PQA_PROJ_WAVES);
 }
 
-template <typename T, typename TI, typename TO, int STAGE, bool MFMA, bool HERM, bool PACK>
-__global__ __launch_bounds__(kThreads, proj_waves(MFMA, std::is_same<T, float>::value, PACK, STAGE))
+// NT: MFMA tiles per dimension, 1 for d <= 16 (one 16 x 16 tile) or 2 for d <= 32 (2 x 2 tiles of 16).
+template <typename T, typename TI, typename TO, int STAGE, bool MFMA, bool HERM, bool PACK, int NT>
+__global__ __launch_bounds__(kThreads, proj_waves(MFMA, std::is_same<T, float>::value, PACK, STAGE, NT))
 void heom_proj_kernel(StageParams p) {
   constexpr bool kDelta = std::is_same<TO, float>::value;  // mixed precision: FP32 RK4 increment
   constexpr bool kPM = kDelta && PACK;  // packed mixed run: two FP32 registers (rk4_upper_m)
+  static_assert(NT == 1 || (NT == 2 && MFMA), "2 x 2 tiles are an MFMA form");
+#ifndef PQA_WIDE_PF
+#define PQA_WIDE_PF 4  // own-matrix elements per lane the wide class prefetches (the rest load in place)
+#endif
+  constexpr int kPF = NT == 1 ? 4 : PQA_WIDE_PF;  // own-matrix elements per lane prefetched (64 kPF)
+  constexpr int kQP = NT == 1 ? 3 : 9;  // packed upper elements per lane, d(d+1)/2 <= 64 kQP
   extern __shared__ __align__(16) char smem[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: ADO indices stay scalar
@@ -777,15 +794,15 @@ void heom_proj_kernel(StageParams p) {
     const int aend = wi.ado0 + wi.nado;
     int nent_n = 0;
     int2 en_n = make_int2(0, 0);
-    cpx_store<TI> xt_n[4];
+    cpx_store<TI> xt_n[kPF];
     auto prefetch = [&](int an) {
       const int r0 = ro_int(p.rowptr, I.rowptr_off + an);
       nent_n = ro_int(p.rowptr, I.rowptr_off + an + 1) - r0;
       en_n = make_int2(0, 0);
       if (lane < nent_n) en_n = p.ent[I.ent_off + r0 + lane];
-      // own matrix: its first 256 elements (all of it for d <= 16)
+      // own matrix: its first 64 kPF elements (all of it for d <= 16 NT)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < kPF; ++q) {
         const int e = lane + 64 * q;
         if (e < d2) xt_n[q] = xi[(int64_t)an * d2 + e];
       }
@@ -797,11 +814,11 @@ void heom_proj_kernel(StageParams p) {
       const int nent = nent_n;
       const int2 en = en_n;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < kPF; ++q) {
         const int e = lane + 64 * q;
         if (e < d2) w.X[e + (int)(((uint32_t)e * dmag) >> 20)] = cvt<T>(mk(xt_n[q].x, xt_n[q].y));
       }
-      for (int e = 256 + lane; e < d2; e += 64) w.X[e + (int)(((uint32_t)e * dmag) >> 20)] = ldc<T>(xi, base + e);
+      for (int e = 64 * kPF + lane; e < d2; e += 64) w.X[e + (int)(((uint32_t)e * dmag) >> 20)] = ldc<T>(xi, base + e);
       // ---- neighbour list in site order, with coefficients
       const int k = en.y & 0xffff, nf = (en.y >> 16) & 0x7f;
       const bool plus = en.y < 0;
@@ -826,7 +843,7 @@ void heom_proj_kernel(StageParams p) {
       const int nval = nent * d;
       // the MFMA class's stage 1 keeps one more load per lane in flight: it has the registers to
       // spare (no FP64 y, one FP32 register read), and its stage time drops by about 3%
-      constexpr int kCh = (MFMA && STAGE == 1) ? PQA_CHUNK_S1 : kChunk;
+      constexpr int kCh = NT == 2 ? PQA_CHUNK_WIDE : (MFMA && STAGE == 1) ? PQA_CHUNK_S1 : kChunk;
       cpx<T> v[kCh];
       auto issue = [&](int pass, int q0) {
 #pragma unroll
@@ -850,10 +867,10 @@ void heom_proj_kernel(StageParams p) {
         }
       };
       issue(0, 0);
-      cpx<T> acc[4];
-      Pre<kDelta> pr[4];
-      PreM pm[4];
-      if (MFMA && !PACK && PQA_RK4_HOIST == 2) {
+      cpx<T> acc[4 * NT * NT];  // MFMA layout: element (16 ti + 4 gk + r, 16 tj + gi) at (ti NT + tj) 4 + r
+      Pre<kDelta> pr[NT == 1 ? 4 : kQP];
+      PreM pm[NT == 1 ? 4 : kQP];
+      if (MFMA && NT == 1 && !PACK && PQA_RK4_HOIST == 2) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           if (4 * gk + r < d && gi < d) pr[r] = rk4_prefetch<STAGE, kDelta>(p, state_ref<PACK>(I, a, 4 * gk + r, gi));
@@ -861,39 +878,66 @@ void heom_proj_kernel(StageParams p) {
       if (a + kThreads / 64 < aend) prefetch(a + kThreads / 64);
       // ---- commutator and damping (while the neighbour loads are in flight)
       if (MFMA) {
-        floatx4 aR = {0.f, 0.f, 0.f, 0.f}, aI = {0.f, 0.f, 0.f, 0.f};
+        floatx4 aR[NT][NT], aI[NT][NT];
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+          for (int tj = 0; tj < NT; ++tj) {
+            aR[ti][tj] = floatx4{0.f, 0.f, 0.f, 0.f};
+            aI[ti][tj] = floatx4{0.f, 0.f, 0.f, 0.f};
+          }
         // H is read from LDS for every ADO rather than kept in registers across the item's ADOs
         // (an opaque zero offset stops the compiler from hoisting the loads out of the ADO loop)
         int hz = 0;
         asm volatile("" : "+v"(hz));
         const cpx<T>* sHa = sH + hz;
-#pragma unroll
-        for (int kc = 0; kc < 4; ++kc) {
+        // 2 x 2 tiles: one k-chunk at a time keeps 16 operands and 32 accumulators live
+        constexpr int kUnrollKC = NT == 1 ? 4 : 1;
+#pragma unroll kUnrollKC
+        for (int kc = 0; kc < 4 * NT; ++kc) {
           const int kk = 4 * kc + gk;
-          const bool ok = gi < d && kk < d;
-          const cpx<T> xb = ok ? w.X[kk * rs + gi] : mk<T>(0, 0);  // B-layout X[kk][gi]
-          const cpx<T> xa = ok ? w.X[gi * rs + kk] : mk<T>(0, 0);  // A-layout X[gi][kk]
-          const cpx<T> ha = ok ? sHa[gi * rs + kk] : mk<T>(0, 0);  // A-layout H[gi][kk]
-          const cpx<T> hb = ok ? sHa[kk * rs + gi] : mk<T>(0, 0);  // B-layout H[kk][gi]
-          const float xbr = (float)xb.x, xbi = (float)xb.y, xar = (float)xa.x, xai = (float)xa.y;
-          const float har = (float)ha.x, hai = (float)ha.y, hbr = (float)hb.x, hbi = (float)hb.y;
-          // + H X
-          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(har, xbr, aR, 0, 0, 0);
-          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(-hai, xbi, aR, 0, 0, 0);
-          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(har, xbi, aI, 0, 0, 0);
-          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(hai, xbr, aI, 0, 0, 0);
-          // - X H
-          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(xar, -hbr, aR, 0, 0, 0);
-          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(xai, hbi, aR, 0, 0, 0);
-          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(xar, -hbi, aI, 0, 0, 0);
-          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(xai, -hbr, aI, 0, 0, 0);
+          float xar[NT], xai[NT], har[NT], hai[NT], xbr[NT], xbi[NT], hbr[NT], hbi[NT];
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            const int g = 16 * t + gi;
+            const bool ok = g < d && kk < d;
+            const cpx<T> xb = ok ? w.X[kk * rs + g] : mk<T>(0, 0);  // B-layout X[kk][g]
+            const cpx<T> xa = ok ? w.X[g * rs + kk] : mk<T>(0, 0);  // A-layout X[g][kk]
+            const cpx<T> ha = ok ? sHa[g * rs + kk] : mk<T>(0, 0);  // A-layout H[g][kk]
+            const cpx<T> hb = ok ? sHa[kk * rs + g] : mk<T>(0, 0);  // B-layout H[kk][g]
+            xbr[t] = (float)xb.x, xbi[t] = (float)xb.y, xar[t] = (float)xa.x, xai[t] = (float)xa.y;
+            har[t] = (float)ha.x, hai[t] = (float)ha.y, hbr[t] = (float)hb.x, hbi[t] = (float)hb.y;
+          }
+#pragma unroll
+          for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj) {
+              floatx4 r = aR[ti][tj], m = aI[ti][tj];
+              // + H X: rows of tile ti from H, columns of tile tj from X
+              r = __builtin_amdgcn_mfma_f32_16x16x4f32(har[ti], xbr[tj], r, 0, 0, 0);
+              r = __builtin_amdgcn_mfma_f32_16x16x4f32(-hai[ti], xbi[tj], r, 0, 0, 0);
+              m = __builtin_amdgcn_mfma_f32_16x16x4f32(har[ti], xbi[tj], m, 0, 0, 0);
+              m = __builtin_amdgcn_mfma_f32_16x16x4f32(hai[ti], xbr[tj], m, 0, 0, 0);
+              // - X H
+              r = __builtin_amdgcn_mfma_f32_16x16x4f32(xar[ti], -hbr[tj], r, 0, 0, 0);
+              r = __builtin_amdgcn_mfma_f32_16x16x4f32(xai[ti], hbi[tj], r, 0, 0, 0);
+              m = __builtin_amdgcn_mfma_f32_16x16x4f32(xar[ti], -hbi[tj], m, 0, 0, 0);
+              m = __builtin_amdgcn_mfma_f32_16x16x4f32(xai[ti], -hbr[tj], m, 0, 0, 0);
+              aR[ti][tj] = r;
+              aI[ti][tj] = m;
+            }
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = 4 * gk + r;
-          acc[r] = mul_mi(mk<T>((T)aR[r], (T)aI[r]));
-          if (i < d && gi < d) cmsub(acc[r], damp, w.X[i * rs + gi]);
-        }
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+          for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int i = 16 * ti + 4 * gk + r, j = 16 * tj + gi;
+              cpx<T>& a2 = acc[(ti * NT + tj) * 4 + r];
+              a2 = mul_mi(mk<T>((T)aR[ti][tj][r], (T)aI[ti][tj][r]));
+              if (i < d && j < d) cmsub(a2, damp, w.X[i * rs + j]);
+            }
       }
       stash(0, 0);
       for (int pass = 0; pass < (HERM ? 1 : 2); ++pass)
@@ -904,10 +948,10 @@ void heom_proj_kernel(StageParams p) {
       // RK4 operands of the lane's four elements, requested together (one round trip instead of
       // one per element); the gather below overlaps their latency
       if (MFMA && PACK && !(kPM && STAGE == 0)) {
-        // packed mode: RK4 operands of this lane's upper elements, packed indices lane + 64q (d <= 16:
-        // at most 136 of them), read in packed order
+        // packed mode: RK4 operands of this lane's upper elements, packed indices lane + 64q (at most
+        // 64 kQP of them), read in packed order
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < kQP; ++q) {
           const int pk = lane + 64 * q;
           if (pk < ((d * (d + 1)) >> 1)) {
             const int64_t g = I.poff + (int64_t)a * I.dp + pk;
@@ -915,7 +959,7 @@ void heom_proj_kernel(StageParams p) {
             else pr[q] = rk4_prefetch<STAGE, kDelta>(p, StateRef{g, false});
           }
         }
-      } else if (MFMA && PQA_RK4_HOIST == 1) {
+      } else if (MFMA && NT == 1 && PQA_RK4_HOIST == 1) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           if (4 * gk + r < d && gi < d) pr[r] = rk4_prefetch<STAGE, kDelta>(p, state_ref<PACK>(I, a, 4 * gk + r, gi));
@@ -946,10 +990,10 @@ void heom_proj_kernel(StageParams p) {
         if (MFMA) {
           // stages 0-2 of a packed mixed run: this lane's upper elements of the FP32 stage input (the
           // own matrix), kept in FP32 and read before the tile takes its place
-          cpx<T> yo[3];
+          cpx<T> yo[kQP];
           if constexpr (kPM && STAGE <= 2) {
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
+            for (int q = 0; q < kQP; ++q) {
               const int pk = lane + 64 * q;
               if (pk < np) {
                 const int ij = sIJ[pk];
@@ -958,15 +1002,19 @@ void heom_proj_kernel(StageParams p) {
             }
           }
           // commutator and damping of every element (MFMA lane layout) through the tile
-          if (gi < d) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-              if (4 * gk + r < d) w.Tt[(4 * gk + r) * rs + gi] = acc[r];
-          }
+          for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+              for (int r = 0; r < 4; ++r) {
+                const int i = 16 * ti + 4 * gk + r, j = 16 * tj + gi;
+                if (i < d && j < d) w.Tt[i * rs + j] = acc[(ti * NT + tj) * 4 + r];
+              }
           wave_sync();
-          cpx<T> au[3];
+          cpx<T> au[kQP];
 #pragma unroll
-          for (int q = 0; q < 3; ++q) {
+          for (int q = 0; q < kQP; ++q) {
             const int pk = lane + 64 * q;
             if (pk < np) {
               const int ij = sIJ[pk], i = ij >> 8, j = ij & 255;
@@ -976,7 +1024,7 @@ void heom_proj_kernel(StageParams p) {
           }
           wave_sync();  // every tile read is done: the tile now takes the stage output
 #pragma unroll
-          for (int q = 0; q < 3; ++q) {
+          for (int q = 0; q < kQP; ++q) {
             const int pk = lane + 64 * q;
             if (pk < np) {
               const int ij = sIJ[pk];
@@ -1034,36 +1082,49 @@ void heom_proj_kernel(StageParams p) {
           st(xout + I.off + base + e, mk<TO>((TO)u.x, (TO)u.y));
         }
       } else if (MFMA) {
-        if (gi < d) {
-          // column term, site gi: element (4gk+r, gi) <- cR_t * conj(R_t[4gk+r])
-          for (int t = w.gs[gi]; t < w.gs[gi + 1]; ++t) {
-            const cpx<T> c = w.cr[t];
-            const cpx<T>* row = (HERM ? w.R : w.Cc) + t * rs + 4 * gk;
+        // tile by tile (one for d <= 16): lane (gk, gi) holds elements (16 ti + 4 gk + r, 16 tj + gi)
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-              // the diagonal element keeps X'(s, s) itself, so that (cL + cR) X'(s, s) is formed exactly
-              // as without the Hermitian shortcut (cL = -cR for the n + e_k terms: no trace drift)
-              if (4 * gk + r < d) cmac(acc[r], c, (HERM && 4 * gk + r != gi) ? conjg(row[r]) : row[r]);
-          }
+        for (int ti = 0; ti < NT; ++ti)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int i = 4 * gk + r;
-            // row term, site i: element (i, gi) <- cL_t * R_t[gi]
-            if (i < d)
-              for (int t = w.gs[i]; t < w.gs[i + 1]; ++t) cmac(acc[r], w.cl[t], w.R[t * rs + gi]);
-          }
+          for (int tj = 0; tj < NT; ++tj) {
+            const int j = 16 * tj + gi, i0 = 16 * ti + 4 * gk;
+            cpx<T>* at = acc + (ti * NT + tj) * 4;
+            if (j < d) {
+              if (NT == 2) {
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int i = 4 * gk + r;
-            if (i < d) {
-              const int e = i * d + gi;
-              const StateRef sr = state_ref<PACK>(I, a, i, gi);
-              if (!PQA_RK4_HOIST) pr[r] = rk4_prefetch<STAGE, kDelta>(p, sr);
-              rk4_store<TO, STAGE, PACK, kDelta>(p, I.off + base + e, I.off + base + gi * d + i, sr, (double)acc[r].x,
-                                         (double)acc[r].y, pr[r], xout, save_ado, tbase + e, tbase + gi * d + i);
+                for (int r = 0; r < 4; ++r)
+                  if (i0 + r < d) pr[r] = rk4_prefetch<STAGE, kDelta>(p, state_ref<PACK>(I, a, i0 + r, j));
+              }
+              // column term, site j: element (i0+r, j) <- cR_t * conj(R_t[i0+r])
+              for (int t = w.gs[j]; t < w.gs[j + 1]; ++t) {
+                const cpx<T> c = w.cr[t];
+                const cpx<T>* row = (HERM ? w.R : w.Cc) + t * rs + i0;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                  // the diagonal element keeps X'(s, s) itself, so that (cL + cR) X'(s, s) is formed exactly
+                  // as without the Hermitian shortcut (cL = -cR for the n + e_k terms: no trace drift)
+                  if (i0 + r < d) cmac(at[r], c, (HERM && i0 + r != j) ? conjg(row[r]) : row[r]);
+              }
+#pragma unroll
+              for (int r = 0; r < 4; ++r) {
+                const int i = i0 + r;
+                // row term, site i: element (i, j) <- cL_t * R_t[j]
+                if (i < d)
+                  for (int t = w.gs[i]; t < w.gs[i + 1]; ++t) cmac(at[r], w.cl[t], w.R[t * rs + j]);
+              }
+#pragma unroll
+              for (int r = 0; r < 4; ++r) {
+                const int i = i0 + r;
+                if (i < d) {
+                  const int e = i * d + j;
+                  const StateRef sr = state_ref<PACK>(I, a, i, j);
+                  if (NT == 1 && !PQA_RK4_HOIST) pr[r] = rk4_prefetch<STAGE, kDelta>(p, sr);
+                  rk4_store<TO, STAGE, PACK, kDelta>(p, I.off + base + e, I.off + base + j * d + i, sr, (double)at[r].x,
+                                             (double)at[r].y, pr[r], xout, save_ado, tbase + e, tbase + j * d + i);
+                }
+              }
             }
           }
-        }
       } else {
         // four elements per lane at a time: their RK4 operands are requested together
         for (int e0 = 0; e0 < d2; e0 += 256) {
@@ -1547,9 +1608,9 @@ struct pyqed_heom_plan {
   int d2max = 1, qmax = 1, Kmax = 1, Lmax = 0;
   int d2pad = 2;
   std::vector<WorkItem> items;  // streamed items by class (generic, projector/VALU, projector/MFMA), then resident
-  int cls_begin[4] = {0, 0, 0, 0};  // class c occupies [cls_begin[c], cls_begin[c+1])
+  int cls_begin[kClasses + 1] = {0};  // class c occupies [cls_begin[c], cls_begin[c+1])
   int n_items_stream = 0;
-  int xr[3][9] = {{0}};
+  int xr[kClasses][9] = {{0}};
   std::vector<int32_t> resident;
   int64_t res_elems = 0, stream_elems = 0;
   int num_cu = 256, max_lds = 65536;
@@ -1575,12 +1636,12 @@ struct pyqed_heom_plan {
   // a class of small instances is not sized by the largest
   struct ProjDims {
     int d2pad = 2, rowbuf = 1, colbuf = 0, Kmax = 1, Lmax = 0;
-  } pd[3];
+  } pd[kClasses];
   bool hermitian = false;
   bool packed = false;     // y, ynew hold upper triangles (PYQED_PLAN_PACKED)
   bool adjoint = false;    // the plan's operator is the adjoint L^H (PYQED_PLAN_ADJOINT)
   int64_t n_state = 0;     // RK4 state elements (size of y): n_elems, or the packed count
-  int64_t cls_state[3] = {0, 0, 0};  // RK4 state elements of the streamed items per class
+  int64_t cls_state[kClasses] = {0};  // RK4 state elements of the streamed items per class
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // the generic class (small d, latency-bound launches) runs its stages on a stream of its own,
@@ -1590,7 +1651,7 @@ struct pyqed_heom_plan {
   // per-launch timing (pyqed_heom_plan_set_timing): event pairs around every class launch
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
-  std::vector<int> ev_tag;  // stage*3 + class of pair k (events 2k, 2k+1)
+  std::vector<int> ev_tag;  // stage * kClasses + class of pair k (events 2k, 2k+1)
 };
 
 namespace {
@@ -1619,7 +1680,7 @@ size_t stream_lds_bytes(const pyqed_heom_plan* P, size_t tsize) {
 
 size_t proj_lds_bytes(const pyqed_heom_plan* P, int c, size_t tsize) {
   const auto& q = P->pd[c];
-  const bool mfma = c == 2 && tsize == sizeof(float);  // the MFMA form keeps its tile in the own-matrix buffer
+  const bool mfma = c >= 2 && tsize == sizeof(float);  // the MFMA forms keep their tile in the own-matrix buffer
   return proj_block_bytes(q.d2pad, q.Kmax, q.Lmax, tsize, P->packed) +
          (kThreads / 64) * proj_wave_bytes(q.d2pad, q.rowbuf, q.colbuf, tsize, P->packed && !mfma);
 }
@@ -1710,14 +1771,14 @@ void retile(pyqed_heom_plan* P, int64_t target, int blocks_per_cu) {
       P->items.push_back(w);
     }
   };
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < kClasses; ++c) {
     P->cls_begin[c] = (int)P->items.size();
     for (int i = 0; i < P->n_inst; ++i)
       if (!P->is_res[i] && P->inst[i].path == c) tile(i);
   }
-  P->cls_begin[3] = P->n_items_stream = (int)P->items.size();
+  P->cls_begin[kClasses] = P->n_items_stream = (int)P->items.size();
   for (int i : P->resident) tile(i);
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < kClasses; ++c) {
     const int b0 = P->cls_begin[c], n = P->cls_begin[c + 1] - b0;
     std::vector<int64_t> pre(n + 1, 0);
     for (int k = 0; k < n; ++k) pre[k + 1] = pre[k] + (int64_t)P->items[b0 + k].nado * P->inst[P->items[b0 + k].inst].d2;
@@ -1911,8 +1972,9 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
       }
       // projector path: at most kMaxEnt neighbour entries per ADO (K plus + min(K, L) minus)
       if (proj && s.K + std::min(s.K, s.L) <= kMaxEnt) {
-        // MFMA commutator (16x16 tiles, d padded to 16) for PQA_MFMA_DMIN <= d <= 16, VALU otherwise
-        I.path = (s.d >= PQA_MFMA_DMIN && s.d <= 16) ? 2 : 1;
+        // MFMA commutator for PQA_MFMA_DMIN <= d <= 16 (one 16 x 16 tile, d padded to 16) and for
+        // 17 <= d <= 32 (2 x 2 tiles, d padded to 32); VALU below
+        I.path = (s.d >= PQA_MFMA_DMIN && s.d <= 16) ? 2 : (s.d > 16 ? 3 : 1);
         I.site_off = (int)sites.size();
         bool sorted_sites = true;
         for (int k = 0; k < s.K; ++k) {
@@ -2045,7 +2107,7 @@ extern "C" int pyqed_heom_plan_layout(const pyqed_heom_plan* P, int64_t* so, int
 
 extern "C" int pyqed_heom_plan_class_state(const pyqed_heom_plan* P, int64_t* state_elems) {
   if (!P || !state_elems) return fail(PYQED_EINVAL, "plan_class_state: null argument");
-  for (int c = 0; c < 3; ++c) state_elems[c] = P->cls_state[c];
+  for (int c = 0; c < kClasses; ++c) state_elems[c] = P->cls_state[c];
   return PYQED_OK;
 }
 
@@ -2058,7 +2120,7 @@ extern "C" int pyqed_heom_plan_set_timing(pyqed_heom_plan* P, int32_t enable) {
 
 extern "C" int pyqed_heom_plan_read_timing(pyqed_heom_plan* P, double* ms, int64_t* launches) {
   if (!P || !ms || !launches) return fail(PYQED_EINVAL, "plan_read_timing: null argument");
-  for (int k = 0; k < 12; ++k) {
+  for (int k = 0; k < 4 * kClasses; ++k) {
     ms[k] = 0.0;
     launches[k] = 0;
   }
@@ -2104,7 +2166,7 @@ extern "C" int pyqed_heom_plan_get_tiling(const pyqed_heom_plan* P, int64_t* tar
 
 extern "C" int pyqed_heom_plan_classes(const pyqed_heom_plan* P, int64_t* items, int64_t* elems) {
   if (!P) return fail(PYQED_EINVAL, "plan_classes: null plan");
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < kClasses; ++c) {
     if (items) items[c] = P->cls_begin[c + 1] - P->cls_begin[c];
     if (elems) {
       elems[c] = 0;
@@ -2268,27 +2330,33 @@ int launch_stage_ti(pyqed_heom_plan* P, const StageParams& p0, hipStream_t st, i
     // the projector kernel keeps neighbour rows (and columns) in LDS; when they do not fit, the
     // generic kernel (which any instance can use) takes the class
     const size_t plds = c == 0 ? 0 : proj_lds_bytes(P, c, sizeof(T));
-    LaunchTimer timer(P, st, 3 * S + c);
+    LaunchTimer timer(P, st, kClasses * S + c);
     constexpr bool F32 = std::is_same<T, float>::value;
     if (c == 0 || plds > (size_t)P->max_lds) {
       rc = launch(heom_stage_kernel<T, TI, T, S, PACK>, grid, stream_lds_bytes(P, sizeof(T)), st, p);
     } else if constexpr (PACK) {  // packed plans are Hermitian
       if (c == 2 && F32)
-        rc = launch(heom_proj_kernel<T, TI, T, S, F32, true, true>, grid, plds, st, p);
+        rc = launch(heom_proj_kernel<T, TI, T, S, F32, true, true, 1>, grid, plds, st, p);
+      else if (c == 3 && F32)
+        rc = launch(heom_proj_kernel<T, TI, T, S, F32, true, true, F32 ? 2 : 1>, grid, plds, st, p);
       else
-        rc = launch(heom_proj_kernel<T, TI, T, S, false, true, true>, grid, plds, st, p);
+        rc = launch(heom_proj_kernel<T, TI, T, S, false, true, true, 1>, grid, plds, st, p);
     } else if (P->hermitian && c == 2 && F32 && P->variant == 1 && pipe_lds_bytes(P, c) <= (size_t)P->max_lds) {
       rc = launch(heom_pipe_kernel<S>, grid, pipe_lds_bytes(P, c), st, p);
     } else if (P->hermitian) {
       if (c == 2 && F32)
-        rc = launch(heom_proj_kernel<T, TI, T, S, F32, true, false>, grid, plds, st, p);
+        rc = launch(heom_proj_kernel<T, TI, T, S, F32, true, false, 1>, grid, plds, st, p);
+      else if (c == 3 && F32)
+        rc = launch(heom_proj_kernel<T, TI, T, S, F32, true, false, F32 ? 2 : 1>, grid, plds, st, p);
       else
-        rc = launch(heom_proj_kernel<T, TI, T, S, false, true, false>, grid, plds, st, p);
+        rc = launch(heom_proj_kernel<T, TI, T, S, false, true, false, 1>, grid, plds, st, p);
     } else {
       if (c == 2 && F32)
-        rc = launch(heom_proj_kernel<T, TI, T, S, F32, false, false>, grid, plds, st, p);
+        rc = launch(heom_proj_kernel<T, TI, T, S, F32, false, false, 1>, grid, plds, st, p);
+      else if (c == 3 && F32)
+        rc = launch(heom_proj_kernel<T, TI, T, S, F32, false, false, F32 ? 2 : 1>, grid, plds, st, p);
       else
-        rc = launch(heom_proj_kernel<T, TI, T, S, false, false, false>, grid, plds, st, p);
+        rc = launch(heom_proj_kernel<T, TI, T, S, false, false, false, 1>, grid, plds, st, p);
     }
     if (rc) return rc;
   }
@@ -2296,7 +2364,7 @@ int launch_stage_ti(pyqed_heom_plan* P, const StageParams& p0, hipStream_t st, i
 }
 
 template <typename T, int S>
-int launch_stage(pyqed_heom_plan* P, const StageParams& p0, hipStream_t st, int c_lo = 0, int c_hi = 3) {
+int launch_stage(pyqed_heom_plan* P, const StageParams& p0, hipStream_t st, int c_lo = 0, int c_hi = kClasses) {
   if (P->packed) return launch_stage_ti<T, S, T, true>(P, p0, st, c_lo, c_hi);
   // unpacked stage 0 reads y itself (FP64)
   return launch_stage_ti<T, S, typename std::conditional<S == 0, double, T>::type, false>(P, p0, st, c_lo, c_hi);
@@ -2380,7 +2448,7 @@ int run_typed(pyqed_heom_plan* P, double2* y, char* ws, const double* field, int
       if (rc) return rc;
     }
     // the generic class on its own stream when other classes share the step
-    const bool split = P->cls_begin[1] > P->cls_begin[0] && P->cls_begin[3] > P->cls_begin[1];
+    const bool split = P->cls_begin[1] > P->cls_begin[0] && P->cls_begin[kClasses] > P->cls_begin[1];
     if (split) {
       PQA_HIP(hipEventRecord(P->ev_fork0, st));
       PQA_HIP(hipStreamWaitEvent(P->side0, P->ev_fork0, 0));
@@ -2392,8 +2460,8 @@ int run_typed(pyqed_heom_plan* P, double2* y, char* ws, const double* field, int
       int rc;
       if (split) {
         if ((rc = launch_step<T>(P, p, x1, x0, save, slot, P->side0, 0, 1))) return rc;
-        if ((rc = launch_step<T>(P, p, x1, x0, save, slot, st, 1, 3))) return rc;
-      } else if ((rc = launch_step<T>(P, p, x1, x0, save, slot, st, 0, 3))) {
+        if ((rc = launch_step<T>(P, p, x1, x0, save, slot, st, 1, kClasses))) return rc;
+      } else if ((rc = launch_step<T>(P, p, x1, x0, save, slot, st, 0, kClasses))) {
         return rc;
       }
     }

@@ -14,8 +14,8 @@ Two routes:
     gamma > 1), and the coefficients then carry the factor 1/lead(denom).
     Pass ``monic=True`` to divide by the leading coefficient (physical
     normalisation).
-  * drude(...) evaluates the Drude-Lorentz + Pade decomposition in closed form
-    (physical normalisation), with no sympy: the route for large batches.
+  * drude(...) and brownian(...) evaluate the Drude-Lorentz and Brownian-oscillator + Pade
+    decompositions in closed form (physical normalisation), with no sympy: the route for large batches.
   * decompose_spectrum_prony(...) and its helpers follow deom.py:21-66, 310-542: C(t) sampled
     from the Pade decomposition and fitted by Prony's method (Hankel eigenvectors, polynomial roots,
     least-squares amplitudes).  Host-only preprocessing, run once per bath.
@@ -367,6 +367,62 @@ def drude(lam, gam, beta, npsd, pade=1):
     return etal, np.conj(etal), np.abs(etal), np.array(expn, np.complex128)
 
 
+def brownian(lam, omega0, gam, beta, npsd, pade=1):
+    """Brownian oscillator J(w) = 2 lam w0^2 gam w / ((w0^2 - w^2)^2 + gam^2 w^2) in closed form
+    (physical normalisation), the decomposition decompose_spectrum_pade makes of it (deom.py:226-307)
+    without sympy.
+
+    The four poles of J are +-Omega +- i gam/2 with Omega = sqrt(w0^2 - gam^2/4) (underdamped), or
+    +-i kappa_1,2 with kappa = gam/2 -+ sqrt(gam^2/4 - w0^2) (overdamped).  Each pole p in the lower half
+    plane gives the exponent i p and the amplitude -2i N(p) / prod_{q != p} (p - q) n_B(beta p), with
+    N(w) = 2 lam w0^2 gam w and n_B the Pade Bose function; each Pade pole xi_j gives the exponent
+    xi_j / beta and the amplitude -2i r_j / beta J(-i xi_j / beta) (the pole sum of the reference).
+    Order as the reference lists them: the complex pair (gam/2 + i Omega, gam/2 - i Omega), whose etar
+    are the conjugates of each other's etal, or the real exponents (kappa_2 >= kappa_1), then the
+    Pade terms.  The critically damped case (gam = 2 w0, a double pole) is rejected.
+    Returns (etal, etar, etaa, expn)."""
+    lam, w0, g = float(lam), float(omega0), float(gam)
+    if not (w0 > 0 and g > 0):
+        raise ValueError("brownian: omega0 and gamma must be positive")
+    disc = w0 * w0 - g * g / 4
+    if abs(disc) <= 1e-12 * w0 * w0:
+        raise ValueError("brownian: critically damped spectrum (gamma = 2 omega0) has a double pole")
+    if disc > 0:
+        om = math.sqrt(disc)
+        lower = [complex(om, -g / 2), complex(-om, -g / 2)]   # exponents g/2 + i om, g/2 - i om
+        poles = np.array([complex(om, g / 2), complex(-om, g / 2)] + lower)
+    else:
+        r = math.sqrt(-disc)
+        k1, k2 = g / 2 - r, g / 2 + r
+        lower = [complex(0, -k2), complex(0, -k1)]              # exponents k2 >= k1
+        poles = np.array([complex(0, k1), complex(0, k2)] + lower)
+    pole, resi = pade_approximation_distribution(npsd, 1, pade)
+
+    def numer(w):
+        return 2.0 * lam * w0 * w0 * g * w
+
+    etal = []
+    for p in lower:
+        others = poles[np.abs(poles - p) > 1e-14 * max(1.0, abs(p))]
+        etal.append(-2j * numer(p) / complex(np.prod(p - others)) * function_bose(p * beta, pole, resi))
+    expn = [1j * p for p in lower]
+    if disc > 0:
+        etar = [np.conj(etal[1]), np.conj(etal[0])]
+        etaa = [math.sqrt(abs(etal[0]) * abs(etal[1]))] * 2
+    else:
+        etar = [np.conj(e) for e in etal]
+        etaa = [abs(e) for e in etal]
+    for j in range(len(pole)):
+        z = -1j * pole[j] / beta
+        e = -2j * resi[j] / beta * numer(z) / complex(np.prod(z - poles))
+        expn.append(complex(pole[j] / beta))
+        etal.append(e)
+        etar.append(np.conj(e))
+        etaa.append(abs(e))
+    return (np.array(etal, np.complex128), np.array(etar, np.complex128), np.array(etaa, np.float64),
+            np.array(expn, np.complex128))
+
+
 class Bath:
     """Bath of one or more spectral densities, drop-in for pyqed.heom.deom.Bath (deom.py:895-942).
 
@@ -422,6 +478,16 @@ class Bath:
         b.expn = np.asarray(expn, np.complex128)
         b.mode = np.asarray(mode, np.int64)
         return b
+
+    @classmethod
+    def brownian(cls, lam, omega0, gam, beta, npsd, mode=None, nbath=1):
+        """nbath identical Brownian-oscillator baths in closed form; mode defaults to bath i -> coupling i."""
+        e = brownian(lam, omega0, gam, beta, npsd)
+        k = len(e[3])
+        etal, etar, etaa, expn = (np.tile(a, nbath) for a in e)
+        if mode is None:
+            mode = np.repeat(np.arange(nbath), k)
+        return cls.from_arrays(etal, etar, etaa, expn, mode)
 
     @classmethod
     def drude(cls, lam, gam, beta, npsd, mode=None, nbath=1):

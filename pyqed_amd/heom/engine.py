@@ -24,6 +24,7 @@ import numpy as np
 from .. import _lib
 
 PRECISIONS = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED, "fp32": _lib.PREC_MIXED}
+NCLASSES = 4  # streamed kernel classes of a plan (pyqed_heom_plan_classes)
 
 
 def _is_diag(m, tol=0.0):
@@ -243,14 +244,16 @@ class HeomEngine:
         return {"target_elems": t.value, "blocks_per_cu": b.value, "variant": self.variant}
 
     def classes(self):
-        it = np.zeros(3, np.int64)
-        el = np.zeros(3, np.int64)
+        """Streamed work items and full-matrix elements per kernel class (0 generic, 1 site projector with
+        VALU commutator, 2 with MFMA on one 16 x 16 tile, 3 with MFMA on 2 x 2 tiles)."""
+        it = np.zeros(NCLASSES, np.int64)
+        el = np.zeros(NCLASSES, np.int64)
         _lib.check(self.L.pyqed_heom_plan_classes(self.plan, it.ctypes.data, el.ctypes.data))
         return dict(items=it.tolist(), elems=el.tolist())
 
     def class_state(self):
         """RK4-state elements of the streamed work per kernel class (packed counts when packed)."""
-        st = np.zeros(3, np.int64)
+        st = np.zeros(NCLASSES, np.int64)
         _lib.check(self.L.pyqed_heom_plan_class_state(self.plan, st.ctypes.data))
         return st.tolist()
 
@@ -259,10 +262,10 @@ class HeomEngine:
         _lib.check(self.L.pyqed_heom_plan_set_timing(self.plan, 1 if enable else 0))
 
     def read_timing(self):
-        """(ms[12], launches[12]) per RK4 stage s and kernel class c at index 3s + c, summed over the
+        """(ms[16], launches[16]) per RK4 stage s and kernel class c at index 4s + c, summed over the
         launches recorded since set_timing / the last read."""
-        ms = np.zeros(12, np.float64)
-        n = np.zeros(12, np.int64)
+        ms = np.zeros(4 * NCLASSES, np.float64)
+        n = np.zeros(4 * NCLASSES, np.int64)
         _lib.check(self.L.pyqed_heom_plan_read_timing(self.plan, ms.ctypes.data, n.ctypes.data))
         return ms.tolist(), n.tolist()
 

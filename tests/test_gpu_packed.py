@@ -34,7 +34,8 @@ def _oracle_step(inst, state, dt):
                                                     mode), dt, 0.0)
 
 
-@pytest.mark.parametrize("d,diag_q", [(5, True), (7, False), (13, False), (16, False), (20, False)])
+@pytest.mark.parametrize("d,diag_q", [(5, True), (7, False), (13, False), (16, False), (17, False), (20, False),
+                                      (24, False), (31, False)])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_packed_one_step_random_hermitian_state(d, diag_q, prec):
     from pyqed_amd.heom.engine import HeomEngine
@@ -59,7 +60,7 @@ def test_packed_batch_matches_full_matrix_plan(prec, tol):
     """One plan with instances of every class, LDS-resident ones included, 60 steps saved every 7:
     packed and full-matrix plans give the same trajectories and final hierarchies."""
     from pyqed_amd.heom.engine import HeomEngine
-    insts = [_inst(d, 2, seed=40 + d) for d in (2, 3, 6, 9, 12, 14, 16, 18)] + [_inst(7, 2, seed=3, diag_q=True)]
+    insts = [_inst(d, 2, seed=40 + d) for d in (2, 3, 6, 9, 12, 14, 16, 18, 25)] + [_inst(7, 2, seed=3, diag_q=True)]
     a = HeomEngine(insts)
     b = HeomEngine(insts, packed=False)
     assert a.packed and not b.packed and a.hermitian and b.hermitian
@@ -97,16 +98,17 @@ def test_packed_needs_hermitian_and_sizes():
 
 def test_launch_timing_counts_every_stage_launch():
     from pyqed_amd.heom.engine import HeomEngine
-    insts = [_inst(d, 2, seed=70 + d) for d in (6, 14)]
+    insts = [_inst(d, 2, seed=70 + d) for d in (6, 14, 20)]
     eng = HeomEngine(insts, resident=False)
     eng.set_timing(True)
     eng.advance(3, 0.01, "mixed")
     ms, n = eng.read_timing()
     eng.set_timing(False)
     items = eng.classes()["items"]
+    nc = len(items)
     for s in range(4):
-        for c in range(3):
-            assert n[3 * s + c] == (3 if items[c] else 0)
-            assert (ms[3 * s + c] > 0) == bool(items[c])
+        for c in range(nc):
+            assert n[nc * s + c] == (3 if items[c] else 0)
+            assert (ms[nc * s + c] > 0) == bool(items[c])
     eng.advance(1, 0.01, "mixed")
     assert sum(eng.read_timing()[1]) == 0  # recording stopped

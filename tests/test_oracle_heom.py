@@ -266,3 +266,72 @@ def test_planted_mistakes_are_caught():
         else:
             out = O.rhs_vec(x, H, Q, hier, b, mode)
         assert np.abs(out - ref).max() > 1e-3, name
+
+
+def test_brownian_closed_form():
+    """bath.brownian (closed form, physical normalisation) against the product's sympy route with
+    monic=True, underdamped and overdamped, and against the reference's own Brownian bath
+    (tests/golden/bath_brownian.npz), which carries the leading coefficient of sympy's normalised
+    denominator as a factor (deom.py:232-233 divides by prod(w - p) as if that denominator were monic)."""
+    import sympy as sp
+    from pyqed_amd.heom.bath import Bath, brownian
+    w = sp.symbols(r"\omega", real=True)
+    g = golden("bath_brownian.npz")
+    lam, w0, gb = 0.5, 1.0, 0.6
+    spe = 2 * lam * w0 ** 2 * gb * w / ((w0 ** 2 - w ** 2) ** 2 + gb ** 2 * w ** 2)
+    re_, im_ = sp.cancel(spe).as_real_imag()
+    _, den = sp.cancel(sp.factor(re_ if im_ == 0 else im_)).as_numer_denom()
+    lead = float(sp.Poly(den, w).LC())
+    e = brownian(lam, w0, gb, 1.0, 2)
+    assert np.abs(e[3] - g["expn"]).max() < 1e-14
+    for k, v in zip(("etal", "etar", "etaa"), e[:3]):
+        assert np.abs(v * lead - g[k]).max() < 1e-14 * np.abs(g[k]).max(), k
+    cases = [(0.5, 1.0, 0.6, 1.0, 2), (0.2, 2.0, 0.5, 0.7, 3), (0.3, 0.8, 0.3, 2.0, 1), (0.4, 1.0, 3.0, 1.0, 2),
+             (0.1, 0.5, 2.5, 1.5, 2)]
+    for lam, w0, gb, beta, npsd in cases:
+        spe = 2 * lam * w0 ** 2 * gb * w / ((w0 ** 2 - w ** 2) ** 2 + gb ** 2 * w ** 2)
+        b = Bath([spe], w, [beta], [npsd], [0] * (2 + npsd), monic=True)
+        e = brownian(lam, w0, gb, beta, npsd)
+        for k, v in zip(("etal", "etar", "etaa", "expn"), e):
+            assert np.abs(getattr(b, k) - v).max() <= 1e-13 * np.abs(v).max(), (lam, w0, gb, k)
+    # sum rule, independent of the reference: Re C(0) = (1/pi) int_0^inf J(w) coth(beta w / 2) dw
+    from scipy.integrate import quad
+    lam, w0, gb, beta = 0.3, 1.2, 0.5, 1.0
+    e = brownian(lam, w0, gb, beta, 40)
+    J = lambda x: 2 * lam * w0 ** 2 * gb * x / ((w0 ** 2 - x ** 2) ** 2 + gb ** 2 * x ** 2)  # noqa: E731
+    ref = quad(lambda x: J(x) / np.tanh(beta * x / 2), 0, np.inf, limit=400)[0] / np.pi
+    assert abs(np.sum(e[0]).real - ref) < 1e-6 * ref
+
+
+def test_brownian_rejects_critical_damping():
+    from pyqed_amd.heom.bath import brownian
+    with pytest.raises(ValueError):
+        brownian(0.2, 1.0, 2.0, 1.0, 1)
+
+
+def test_diagonal_coupling_form_equals_rhs_vec():
+    """oracle.heom.rhs_vec_diag (the all-core CPU baseline's right-hand side) equals rhs_vec."""
+    rng = np.random.default_rng(12)
+    d, L = 5, 3
+    hier = O.Hierarchy(2 * d, L)
+    bath = O.drude_bath(0.3, 0.8, 1.1, 1)
+    bath = tuple(np.tile(a, d) for a in bath)
+    mode = np.repeat(np.arange(d), 2)
+    qd = rng.normal(size=(d, d))
+    Q = [np.diag(q).astype(complex) for q in qd]
+    A = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+    H = A + A.conj().T
+    rho = rng.normal(size=(hier.nmax, d, d)) + 1j * rng.normal(size=(hier.nmax, d, d))
+    a = O.rhs_vec(rho, H, Q, hier, bath, mode)
+    b = O.rhs_vec_diag(rho, H, O.DiagTerms(hier, bath, mode, qd))
+    assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
+
+
+def test_vectorised_tables_equal_loop_restatement():
+    """The array-built tables the oracle's Hierarchy uses equal the loop restatement of gen_keys and
+    hash_plus / hash_minus (deom.py:555-638), which the reference's own tables pin."""
+    for K, L in [(1, 4), (2, 5), (3, 2), (4, 3), (6, 3), (5, 0), (0, 3), (8, 2), (10, 3)]:
+        keys = O.gen_keys(K, L)
+        plus, minus = O.neighbour_tables(keys, L)
+        k2, p2, m2 = O.tables_vectorised(K, L)
+        assert np.array_equal(keys, k2) and np.array_equal(plus, p2) and np.array_equal(minus, m2), (K, L)
