@@ -5,25 +5,29 @@ depth 2..4, K = 2d Drude + Pade exponentials, dt = 0.01) advanced together in
 one plan with the FP32-right-hand-side / FP64-state kernels.  A "step" is one
 RK4 step of every instance.
 
-Printed JSON line (rank 0):
-  value        whole-batch ADO-updates/s of the mixed-precision batched path
-  fp64_plain   ADO-updates/s of the plain FP64 kernel run one instance per
-               plan (same instances, same steps), and the ratio value/that
-  error_vs_fp64  max-abs population / coherence difference of every
-               instance's reduced density matrix after warmup+steps steps
-               against that FP64 run
-  roofline     the dominant stage kernel (the kernel-class launch with the
-               longest average time, timed per launch with HIP events on its
-               stream): algorithmic HBM bytes per launch / average launch time,
-               vs 8 TB/s; `step` gives the same for all stage kernels of a step
-  cpu_baseline numpy FP64 restatement (the oracle), one thread, on a stratified
-               sample of the same instances
+Printed JSON line:
+  value          whole-batch ADO-updates/s of the mixed-precision batched path
+  fp64_batched   ADO-updates/s of the same batched kernels in plain FP64 (one plan over every
+                 instance, chunked only if it does not fit the device), and ratio_vs_fp64_batched
+  fp64_plain     ADO-updates/s of the plain FP64 kernels run one instance per plan (same
+                 instances, same steps), and ratio_vs_fp64_plain
+  error_vs_fp64  max-abs population / coherence difference of rho between the mixed and plain FP64
+                 runs over the workload's full step count (2000 for the batch) on a seeded sample of
+                 64 instances covering every (d, depth) stratum; after_timed_steps: the same over
+                 every instance after the timed steps
+  roofline       the dominant stage kernel (the kernel-class launch with the longest average time,
+                 timed per launch with HIP events on its stream): algorithmic HBM bytes per launch /
+                 average launch time, vs 8 TB/s; `step` gives the same for all stage kernels of a
+                 step; `traffic` the measured HBM bytes per launch from the committed counter summary
+                 when it was taken at this workload, state layout and tiling
+  cpu_baseline   numpy FP64 restatement (the oracle) on the host's cores (R1: all cores, vectorised;
+                 R0: one core, the reference's per-ADO loop) on a stratified sample, with the CPU model
 
-usage: python bench.py [--steps K] [--warmup W] [--workload c3|c2|c5|c1|legacy]
+usage: python bench.py [--steps K] [--warmup W] [--workload c3|c2|c5|c1|c4|lindblad|legacy|...]
 
---workload legacy measures the legacy single-exponential HEOM of oqs.HEOMSolver instead: a
-temperature x reorganization sweep of the spin-boson example, one Euler step of every instance per
-step, roofline against the FP64 peak.
+Other workloads: c4 (Lindblad adjoint gradients), lindblad (driven Lindblad forward), legacy /
+legacy-rk4 (oqs.HEOMSolver), spectral (eigen-decomposition spectral sums), krylov (HEOM spectra),
+heom-adjoint (streamed HEOM gradients).
 """
 import argparse
 import json
@@ -68,7 +72,9 @@ def alg_bytes(stage, ts, F):
 NX = {False: (1, 2, 2, 1), True: (2, 2, 2, 2)}
 SB = {("fp64", False): (32, 48, 48, 32), ("fp64", True): (32, 48, 48, 32),
       ("mixed", False): (24, 32, 32, 40), ("mixed", True): (8, 16, 24, 40)}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "heom_proj_kernels_exciton4096_traffic.json")
+# committed per-kernel HBM traffic summaries (bench.py --traffic-from), per workload
+TRAFFIC_FILES = {"c3": os.path.join(ROOT, "profiles", "heom_proj_kernels_exciton4096_traffic.json"),
+                 "c5": os.path.join(ROOT, "profiles", "heom_proj_kernels_chain24_traffic.json")}
 
 
 def stage_bytes(stage, ts, F, S, packed):
@@ -259,17 +265,76 @@ def kernel_name(stage, cls, precision, eng):
     return f"heom_proj_kernel<{T}, {TI}, {T}, {stage}, {mfma}, {herm}, {pack}, {nt}>"
 
 
-def measured_traffic(name, workload, packed):
+def measured_traffic(name, key, workload, packed, tiling):
     """HBM bytes per launch of `name` (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the committed
-    counter summary of the same workload and state layout, or None when there is none."""
-    if not os.path.exists(TRAFFIC_FILE):
+    counter summary of workload `key`, or None unless that summary was taken on the same workload,
+    state layout and work-queue tiling as this run."""
+    path = TRAFFIC_FILES.get(key)
+    if path is None or not os.path.exists(path):
         return None
-    with open(TRAFFIC_FILE) as f:
+    with open(path) as f:
         t = json.load(f)
     if t.get("workload") != workload or bool(t.get("packed", False)) != bool(packed):
         return None
+    if t.get("tiling") != {k: tiling[k] for k in ("target_elems", "blocks_per_cu", "variant")}:
+        return None
     k = t["per_dispatch"].get(name)
     return None if k is None else k["fetch_bytes"] + k["write_bytes"]
+
+
+def _pmc_rows(d):
+    """(kernel name, counter value) of every dispatch in the rocprofv3 --pmc CSV files under d."""
+    import csv
+    import glob
+    import re
+    out = []
+    for path in glob.glob(os.path.join(d, "**", "*counter_collection*.csv"), recursive=True):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                name = r.get("Kernel_Name") or r.get("Kernel-Name") or ""
+                name = re.sub(r"^void ", "", name).replace("(anonymous namespace)::", "")
+                depth, cut = 0, len(name)
+                for i, ch in enumerate(name):  # drop the argument list after the template arguments
+                    if ch == "<":
+                        depth += 1
+                    elif ch == ">":
+                        depth -= 1
+                    elif ch == "(" and depth == 0:
+                        cut = i
+                        break
+                out.append((name[:cut].strip(), float(r["Counter_Value"])))
+    return out
+
+
+def traffic_from(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, bench_json, copy_mib=1024, key="c3"):
+    """Per-dispatch HBM bytes of every kernel from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs (KiB per
+    dispatch), each scaled by what the same counter reads for a device copy of known size
+    (bench.py --copy-calibration), written to the traffic summary together with the workload, state
+    layout and tiling of the profiled run (bench_json: its printed JSON line)."""
+    def per_kernel(d):
+        acc = {}
+        for name, v in _pmc_rows(d):
+            acc.setdefault(name, []).append(v)
+        return {k: float(np.mean(v)) for k, v in acc.items()}
+
+    def copy_kib(d):
+        vals = [v for n, v in _pmc_rows(d) if "copyBuffer" in n or "copy" in n.lower()]
+        return max(vals) if vals else None
+
+    with open(bench_json) as f:
+        b = json.loads([ln for ln in f if ln.startswith("{")][-1])
+    nbytes = copy_mib * (1 << 20)  # bench.py --copy-calibration copy_mib: one copy of that many bytes
+    fs = nbytes / (copy_kib(copy_fetch_dir) * 1024.0)
+    ws = nbytes / (copy_kib(copy_write_dir) * 1024.0)
+    fk, wk = per_kernel(fetch_dir), per_kernel(write_dir)
+    t = {"workload": b["config"]["workload"], "packed": b["state_layout"].startswith("packed"),
+         "tiling": {k: b["tiling"][k] for k in ("target_elems", "blocks_per_cu", "variant")},
+         "fetch_scale": fs, "write_scale": ws,
+         "per_dispatch": {k: {"fetch_bytes": fk[k] * 1024.0 * fs, "write_bytes": wk.get(k, 0.0) * 1024.0 * ws}
+                          for k in fk}}
+    with open(TRAFFIC_FILES[key], "w") as f:
+        json.dump(t, f, indent=1)
+    print(json.dumps({"written": TRAFFIC_FILES[key], "kernels": len(fk), "fetch_scale": fs, "write_scale": ws}))
 
 
 def stream_probe(torch):
@@ -713,6 +778,220 @@ def bench_krylov(args, torch):
     print(json.dumps(res), flush=True)
 
 
+# Full-length accuracy check per workload: the step count each configuration is quoted at.
+FULL_STEPS = {"c1": 1000, "c2": 1000, "c3": 2000, "c5": 5000}
+
+
+def error_sample(insts, n=64, seed=11):
+    """Indices of a seeded accuracy sample: one instance of every (d, depth) stratum, every instance of
+    the largest d at the largest depth (up to 4), then seeded others up to n."""
+    strata = _strata(insts)
+    idx = sorted(strata.values())
+    dmax = max(p["H"].shape[0] for p in insts)
+    lmax = max(p["L"] for p in insts if p["H"].shape[0] == dmax)
+    big = [i for i, p in enumerate(insts) if p["H"].shape[0] == dmax and p["L"] == lmax]
+    for i in big[:4]:
+        if i not in idx:
+            idx.append(i)
+    rest = [i for i in np.random.default_rng(seed).permutation(len(insts)) if i not in idx]
+    idx += [int(i) for i in rest[:max(0, n - len(idx))]]
+    return sorted(idx)
+
+
+def full_length_error(insts, dt, nt, torch, n=64, save_every=20):
+    """Mixed (packed, FP32 right-hand side) against plain FP64, both from rho0 over nt steps, on the
+    seeded sample: max-abs population and coherence difference of rho over every saved time."""
+    from pyqed_amd.heom.engine import HeomEngine
+    idx = error_sample(insts, n) if len(insts) > 1 else [0]
+    sub = [insts[i] for i in idx]
+    traj = {}
+    t_leg = {}
+    for prec in ("mixed", "fp64"):
+        eng = HeomEngine(sub)
+        t0 = time.perf_counter()
+        traj[prec] = eng.run(dt, nt, precision=prec, save_every=save_every)
+        t_leg[prec] = time.perf_counter() - t0
+        eng.close()
+        del eng
+        torch.cuda.empty_cache()
+    pops, cohs = zip(*[max_errs(a, b) for a, b in zip(traj["mixed"], traj["fp64"])])
+    strata = {(p["H"].shape[0], p["L"]) for p in sub}
+    worst = int(np.argmax(pops))
+    return {"population": max(pops), "coherence": max(cohs), "steps": nt, "save_every": save_every,
+            "instances": len(sub), "strata": len(strata), "d_max": max(d for d, _ in strata),
+            "worst_instance": {"index": idx[worst], "d": sub[worst]["H"].shape[0], "depth": sub[worst]["L"],
+                               "population": pops[worst]},
+            "seconds": t_leg, "reference": "the same kernels in plain FP64 (packed state, FP64 stage inputs and "
+                                           "RK4 register), which the golden tests pin to the reference at 1e-12"}
+
+
+def fp64_batched(insts, dt, warmup, steps, torch):
+    """Plain FP64 with the same batched kernels as the mixed leg (one plan over all instances, packed
+    state when Hermitian): time `steps` steps after `warmup`.  When one plan does not fit the device,
+    the instances run in chunks and the chunk times add."""
+    from pyqed_amd.heom.engine import HeomEngine
+    chunks = [insts]
+    while True:
+        try:
+            total, ados, finals = 0.0, 0, []
+            for ch in chunks:
+                eng = HeomEngine(ch)
+                eng.advance(warmup, dt, "fp64")
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                eng.advance(steps, dt, "fp64")
+                torch.cuda.synchronize()
+                total += time.perf_counter() - t0
+                finals += eng.reduced_states()
+                ados += int(eng.n_ados)
+                eng.close()
+                del eng
+                torch.cuda.empty_cache()
+            return ados * steps / total, total, finals, len(chunks)
+        except (torch.cuda.OutOfMemoryError, RuntimeError) as e:
+            if "memory" not in str(e).lower() or len(chunks[0]) < 2:
+                raise
+            torch.cuda.empty_cache()
+            chunks = [c for ch in chunks for c in (ch[:len(ch) // 2], ch[len(ch) // 2:])]
+            print(f"[bench] fp64 batched leg: out of memory, retrying in {len(chunks)} chunks", file=sys.stderr,
+                  flush=True)
+
+
+def bench_heom(args, torch):
+    """ADO-updates/s of DEOMSolver's propagation (pyqed/heom/deom.py:725-766) for one HEOM workload."""
+    from pyqed_amd.heom.engine import HeomEngine
+    insts, dt, desc = instances(args.workload, args.n_inst)
+    print(f"[bench] workload {args.workload}: {len(insts)} instance(s); building plan", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    eng = HeomEngine(insts, packed=False if args.state == "full" else None)
+    if args.tiling:
+        eng.autotune_cache = False  # an explicit tiling, e.g. to profile at the tiling another run used
+        eng.set_tiling(*(int(x) for x in args.tiling.split(",")))
+    torch.cuda.synchronize()
+    t_plan = time.perf_counter() - t0
+    st = eng.stats()
+    print(f"[bench] plan {t_plan:.1f}s: {eng.n_ados} ADOs, {eng.n_elems} elements, {st}", file=sys.stderr,
+          flush=True)
+    eng.advance(args.warmup, dt, args.precision)
+    torch.cuda.synchronize()
+    print(f"[bench] warmup done; state at {eng.y.data_ptr():#x}, workspace at {eng._ws.data_ptr():#x}",
+          file=sys.stderr, flush=True)
+    if args.graph:
+        eng.advance_graph(args.graph, dt, args.precision, chunk=args.graph)  # capture (+ one replay) untimed
+        torch.cuda.synchronize()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    timing = not args.graph and not args.no_launch_timing
+    if timing:
+        eng.set_timing(True)  # event pair around every kernel-class launch, on the launch stream
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evs[0].record()
+    if args.graph:
+        for s in range(0, args.steps, args.graph):
+            eng.advance_graph(min(args.graph, args.steps - s), dt, args.precision, chunk=args.graph)
+        evs[-1].record()
+    else:
+        # one call for all timed steps, as DEOMSolver.run makes (a packed plan rebuilds the stage-0
+        # input from y once per call)
+        eng.advance(args.steps, dt, args.precision)
+        evs[-1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = [evs[0].elapsed_time(evs[-1]) / args.steps]
+    tiling = eng.tiling()
+    tiling_cached = bool(getattr(eng, "_tuned_hit", False))
+    value = eng.n_ados * args.steps / wall
+    print(f"[bench] timed {args.steps} steps in {wall:.3f}s", file=sys.stderr, flush=True)
+    ts = 8 if args.precision == "mixed" else 16
+    F = eng.classes()["elems"]
+    S = eng.class_state()
+    nc = len(F)
+    step_bytes = sum(alg_bytes(s, ts, F[c]) for s in range(4) for c in range(nc))
+    step_layout = sum(stage_bytes(s, ts, F[c], S[c], eng.packed) for s in range(4) for c in range(nc))
+    t_step = float(np.mean(step_ms)) / 1e3
+    roofline = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
+                "bytes_model": "classic three-register RK4, %d B per full-matrix element per step"
+                               % sum(ALG["mixed" if ts == 8 else "fp64"]),
+                "step": {"bytes": step_bytes, "ms": t_step * 1e3, "achieved": step_bytes / t_step / 1e9,
+                         "frac": step_bytes / t_step / 1e9 / HBM_PEAK_GBS, "layout_bytes": step_layout}}
+    avg = {}
+    if timing:
+        ms, launches = eng.read_timing()
+        eng.set_timing(False)
+        avg = {k: ms[k] / launches[k] for k in range(len(ms)) if launches[k]}
+    if avg:  # no streamed launches when every instance runs LDS-resident
+        k = max(avg, key=avg.get)
+        s_dom, c_dom = divmod(k, nc)
+        b = alg_bytes(s_dom, ts, F[c_dom])
+        bl = stage_bytes(s_dom, ts, F[c_dom], S[c_dom], eng.packed)
+        name = kernel_name(s_dom, c_dom, args.precision, eng)
+        roofline.update({"kernel": name, "stage": s_dom, "class": c_dom, "launch_ms": avg[k],
+                         "bytes_per_launch": b, "achieved": b / (avg[k] / 1e3) / 1e9,
+                         "frac": b / (avg[k] / 1e3) / 1e9 / HBM_PEAK_GBS,
+                         "layout_bytes_per_launch": bl, "layout_achieved": bl / (avg[k] / 1e3) / 1e9,
+                         "launch_ms_all": {f"s{kk // nc}c{kk % nc}": v for kk, v in sorted(avg.items())}})
+        roofline["traffic"] = measured_traffic(name, args.workload, desc, eng.packed, tiling)
+    else:
+        roofline.update({"kernel": "all stage kernels of a step" if eng.stats()["stream_items"] else
+                         "heom_resident_kernel (every instance LDS-resident)",
+                         "achieved": roofline["step"]["achieved"], "frac": roofline["step"]["frac"]})
+    mixed_final = eng.reduced_states()
+    packed = eng.packed
+    eng.close()
+    del eng
+    torch.cuda.empty_cache()
+
+    mixed = args.precision == "mixed"
+    out = {
+        "metric": METRIC, "value": value, "unit": "ADO-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": ("f32 right-hand side and stage inputs, f32 RK4 increment registers, f64 state" if mixed
+                  else "f64"),
+        "data": ("synthetic (seeded numpy.random.default_rng; closed-form Drude + Pade baths in physical "
+                 "normalisation: the reference's Bath would carry a factor 1/gamma^2 on every eta of an instance "
+                 "with gamma > 1)"),
+        "config": {"workload": desc, "instances": len(insts), "precision": args.precision},
+        "roofline": roofline,
+        "plan_build_s": t_plan, "plan": st, "tiling": tiling, "tiling_source": "explicit" if args.tiling else (
+            "autotuner cache" if tiling_cached else "automatic"), "graph_chunk": args.graph or None,
+        "device": torch.cuda.get_device_name(), 
+        "state_layout": "packed Hermitian (upper triangles)" if packed else "full matrices",
+    }
+    from math import comb
+    out["config"]["ados"] = int(sum(comb(len(i["bath"][4]) + i["L"], i["L"]) for i in insts))
+    out["config"]["elements"] = int(st["stream_elems"] + st["resident_elems"])
+    pre = args.warmup + (args.graph or 0)  # steps the mixed run made before its final state
+    if mixed and not args.no_fp64:
+        print("[bench] fp64 batched leg", file=sys.stderr, flush=True)
+        v64b, t64b, finals_b, nch = fp64_batched(insts, dt, pre, args.steps, torch)
+        out["fp64_batched"] = {"value": v64b, "unit": "ADO-updates/s", "seconds": t64b, "chunks": nch,
+                               "kernel": "the same batched stage kernels in plain FP64 (one plan over all "
+                                         "instances, packed state, FP64 stage inputs and RK4 register)"}
+        out["ratio_vs_fp64_batched"] = value / v64b
+        pops, cohs = zip(*[max_errs(a, b) for a, b in zip(mixed_final, finals_b)])
+        short = {"population": max(pops), "coherence": max(cohs), "steps": pre + args.steps,
+                 "instances": len(insts), "reference": "fp64_batched"}
+        if not args.no_fp64_plain:
+            print("[bench] fp64 plain leg", file=sys.stderr, flush=True)
+            v64, t64, _ = fp64_plain(insts, dt, pre, args.steps, torch)
+            out["fp64_plain"] = {"value": v64, "unit": "ADO-updates/s", "seconds": t64,
+                                 "kernel": "FP64 stage kernels (projector or generic class as the instance allows), "
+                                           "one instance per plan, full-matrix state, no LDS residency"}
+            out["ratio_vs_fp64_plain"] = value / v64
+        if not args.no_full_error and args.workload in FULL_STEPS:
+            nt = FULL_STEPS[args.workload]
+            print(f"[bench] full-length accuracy leg ({nt} steps)", file=sys.stderr, flush=True)
+            out["error_vs_fp64"] = full_length_error(insts, dt, nt, torch)
+            out["error_vs_fp64"]["after_timed_steps"] = short
+        else:
+            out["error_vs_fp64"] = short
+    if not args.no_cpu:
+        print("[bench] cpu baseline leg", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(insts, dt)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -726,8 +1005,18 @@ def main():
                     help="RK4 state layout: auto = upper triangles for Hermitian hierarchies, full = full matrices")
     ap.add_argument("--no-launch-timing", action="store_true",
                     help="no per-launch HIP events (roofline from the step time only)")
-    ap.add_argument("--no-fp64-plain", action="store_true")
+    ap.add_argument("--no-fp64-plain", action="store_true", help="skip the one-instance-per-plan FP64 leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip every FP64 leg (and the accuracy legs)")
+    ap.add_argument("--no-full-error", action="store_true",
+                    help="skip the full-length mixed-vs-FP64 accuracy leg on the seeded sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic-from", nargs=5, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR",
+                                                       "BENCH_JSON"),
+                    help="only convert rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of a bench command (and of "
+                         "--copy-calibration) into the per-kernel traffic summary")
+    ap.add_argument("--copy-mib", type=int, default=1024, help="size of the --copy-calibration run (MiB)")
+    ap.add_argument("--tiling", default=None, metavar="TARGET,BLOCKS_PER_CU,VARIANT",
+                    help="work-queue tiling of the mixed leg instead of the autotuner's cached choice")
     ap.add_argument("--graph", type=int, default=0, metavar="CHUNK",
                     help="time graph replays of CHUNK captured steps instead of eager launches")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -775,113 +1064,10 @@ def main():
         bench_krylov(args, torch)
         return
 
-    insts, dt, desc = instances(args.workload, args.n_inst)
-    print(f"[bench] workload {args.workload}: {len(insts)} instance(s); building plan", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    eng = HeomEngine(insts, packed=False if args.state == "full" else None)
-    torch.cuda.synchronize()
-    t_plan = time.perf_counter() - t0
-    st = eng.stats()
-    print(f"[bench] plan {t_plan:.1f}s: {eng.n_ados} ADOs, {eng.n_elems} elements, {st}", file=sys.stderr,
-          flush=True)
-    eng.advance(args.warmup, dt, args.precision)
-    torch.cuda.synchronize()
-    print(f"[bench] warmup done; state at {eng.y.data_ptr():#x}, workspace at {eng._ws.data_ptr():#x}",
-          file=sys.stderr, flush=True)
-    if args.graph:
-        eng.advance_graph(args.graph, dt, args.precision, chunk=args.graph)  # capture (+ one replay) untimed
-        torch.cuda.synchronize()
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    timing = not args.graph and not args.no_launch_timing
-    if timing:
-        eng.set_timing(True)  # event pair around every kernel-class launch, on the launch stream
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    evs[0].record()
-    if args.graph:
-        for s in range(0, args.steps, args.graph):
-            eng.advance_graph(min(args.graph, args.steps - s), dt, args.precision, chunk=args.graph)
-        evs[-1].record()
-    else:
-        # one call for all timed steps, as DEOMSolver.run makes (a packed plan rebuilds the stage-0
-        # input from y once per call)
-        eng.advance(args.steps, dt, args.precision)
-        evs[-1].record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    step_ms = [evs[0].elapsed_time(evs[-1]) / args.steps]
-    tiling = eng.tiling()
-    value = eng.n_ados * args.steps / wall
-    print(f"[bench] timed {args.steps} steps in {wall:.3f}s", file=sys.stderr, flush=True)
-    ts = 8 if args.precision == "mixed" else 16
-    F = eng.classes()["elems"]
-    S = eng.class_state()
-    nc = len(F)
-    step_bytes = sum(alg_bytes(s, ts, F[c]) for s in range(4) for c in range(nc))
-    step_layout = sum(stage_bytes(s, ts, F[c], S[c], eng.packed) for s in range(4) for c in range(nc))
-    t_step = float(np.mean(step_ms)) / 1e3
-    roofline = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
-                "bytes_model": "classic three-register RK4, %d B per full-matrix element per step"
-                               % sum(ALG["mixed" if ts == 8 else "fp64"]),
-                "step": {"bytes": step_bytes, "ms": t_step * 1e3, "achieved": step_bytes / t_step / 1e9,
-                         "frac": step_bytes / t_step / 1e9 / HBM_PEAK_GBS, "layout_bytes": step_layout}}
-    avg = {}
-    if timing:
-        ms, launches = eng.read_timing()
-        eng.set_timing(False)
-        avg = {k: ms[k] / launches[k] for k in range(len(ms)) if launches[k]}
-    if avg:  # no streamed launches when every instance runs LDS-resident
-        k = max(avg, key=avg.get)
-        s_dom, c_dom = divmod(k, nc)
-        b = alg_bytes(s_dom, ts, F[c_dom])
-        bl = stage_bytes(s_dom, ts, F[c_dom], S[c_dom], eng.packed)
-        name = kernel_name(s_dom, c_dom, args.precision, eng)
-        roofline.update({"kernel": name, "stage": s_dom, "class": c_dom, "launch_ms": avg[k],
-                         "bytes_per_launch": b, "achieved": b / (avg[k] / 1e3) / 1e9,
-                         "frac": b / (avg[k] / 1e3) / 1e9 / HBM_PEAK_GBS,
-                         "layout_bytes_per_launch": bl, "layout_achieved": bl / (avg[k] / 1e3) / 1e9,
-                         "launch_ms_all": {f"s{kk // nc}c{kk % nc}": v for kk, v in sorted(avg.items())}})
-        roofline["traffic"] = measured_traffic(name, desc, eng.packed)
-    else:
-        roofline.update({"kernel": "all stage kernels of a step" if eng.stats()["stream_items"] else
-                         "heom_resident_kernel (every instance LDS-resident)",
-                         "achieved": roofline["step"]["achieved"], "frac": roofline["step"]["frac"]})
-    mixed_final = eng.reduced_states()
-    packed = eng.packed
-    eng.close()
-    del eng
-    torch.cuda.empty_cache()
-
-    out = {
-        "metric": METRIC, "value": value, "unit": "ADO-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 rhs / f64 state" if args.precision == "mixed" else "f64",
-        "data": "synthetic (seeded numpy.random.default_rng; closed-form Drude + Pade baths)",
-        "config": {"workload": desc, "instances": len(insts), "precision": args.precision},
-        "roofline": roofline,
-        "plan_build_s": t_plan, "plan": st, "tiling": tiling, "graph_chunk": args.graph or None,
-        "state_layout": "packed Hermitian (upper triangles)" if packed else "full matrices",
-    }
-    from math import comb
-    out["config"]["ados"] = int(sum(comb(len(i["bath"][4]) + i["L"], i["L"]) for i in insts))
-    out["config"]["elements"] = int(st["stream_elems"] + st["resident_elems"])
-    if not args.no_fp64_plain and args.precision == "mixed":
-        print("[bench] fp64 plain leg", file=sys.stderr, flush=True)
-        # the mixed run advanced warmup (+ the captured graph chunk, replayed once untimed) + steps
-        pre = args.warmup + (args.graph or 0)
-        v64, t64, finals = fp64_plain(insts, dt, pre, args.steps, torch)
-        pops, cohs = zip(*[max_errs(a, b) for a, b in zip(mixed_final, finals)])
-        out["fp64_plain"] = {"value": v64, "unit": "ADO-updates/s", "seconds": t64,
-                             "kernel": "FP64 stage kernels (projector or generic class as the instance allows), "
-                                       "one instance per plan, full-matrix state, no LDS residency"}
-        out["ratio_vs_fp64_plain"] = value / v64
-        out["error_vs_fp64"] = {"population": max(pops), "coherence": max(cohs),
-                                "steps": pre + args.steps, "instances": len(insts)}
-    if not args.no_cpu:
-        print("[bench] cpu baseline leg", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(insts, dt)
-    print(json.dumps(out), flush=True)
+    if args.traffic_from:
+        traffic_from(*args.traffic_from, copy_mib=args.copy_mib, key=args.workload)
+        return
+    bench_heom(args, torch)
 
 
 if __name__ == "__main__":

@@ -18,17 +18,25 @@ from .. import _lib
 
 TUNING_FILE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tuning.json")
 TARGETS = (2048, 4096, 8192, 16384)
-BLOCKS_PER_CU = (4, 8, 16)
-VARIANTS = (0, 1)
+BLOCKS_PER_CU = (4, 8, 12, 16)
+VARIANTS = (0, 1)  # 1, the pipelined MFMA-class kernel, exists for Hermitian full-matrix plans only
+
+
+def signature_of(arch, cus, precision, hermitian, dense, shapes, packed):
+    """Cache key of a workload: device architecture and CU count, precision, plan flags and every
+    instance's (d, K, L)."""
+    key = [arch, int(cus), precision, bool(hermitian), bool(dense), [list(map(int, t)) for t in shapes]]
+    if packed:
+        key.append("packed")  # packed Hermitian state: its own entry
+    return hashlib.sha1(json.dumps(key).encode()).hexdigest()[:20]
 
 
 def signature(engine, precision):
-    torch = engine.torch
-    key = [torch.cuda.get_device_name(engine.device), precision, engine.hermitian, engine.dense,
-           [(p["d"], len(p["mode"]), p["L"]) for p in engine.prep]]
-    if getattr(engine, "packed", False):
-        key.append("packed")  # packed Hermitian state: its own entry
-    return hashlib.sha1(json.dumps(key).encode()).hexdigest()[:20]
+    """The device enters through its gfx architecture and CU count, which stay the same under a
+    profiler (the marketing name reads empty there)."""
+    props = engine.torch.cuda.get_device_properties(engine.device)
+    return signature_of(props.gcnArchName, props.multi_processor_count, precision, engine.hermitian, engine.dense,
+                        [(p["d"], len(p["mode"]), p["L"]) for p in engine.prep], getattr(engine, "packed", False))
 
 
 def load_cache(path=TUNING_FILE):
@@ -54,6 +62,8 @@ def tune(engine, dt, precision="mixed", targets=TARGETS, blocks_per_cu=BLOCKS_PE
     with milliseconds per step."""
     torch = engine.torch
     timings = {}
+    if engine.packed or not engine.hermitian:
+        variants = tuple(v for v in variants if v == 0) or (0,)
     for v in variants:
         for t in targets:
             for b in blocks_per_cu:

@@ -295,6 +295,7 @@ class HeomEngine:
             hit = autotune.lookup(self, precision)
             if hit is not None:
                 self.set_tiling(*hit)
+                self._tuned_hit = True
         ws = self.workspace(prec)
         _lib.check(self.L.pyqed_heom_run(
             self.plan, prec, self.y.data_ptr(), ws.data_ptr(), ws.numel(),
