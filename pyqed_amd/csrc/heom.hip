@@ -249,10 +249,11 @@ __device__ __forceinline__ cpx<float> ldc<float, cpx<double>>(const cpx<double>*
 // Right-hand side of element (i, j) of ADO `a`.
 //   own : the ADO's own d x d matrix (LDS), in T
 //   xin : base of the instance's stage input (ADO 0), indexable by ldc
+//   rowp, entp: the instance's row pointers and neighbour entries (global memory, or LDS copies)
 template <typename T, typename OS, typename XS>
 __device__ __forceinline__ cpx<T> element_rhs(const StageParams& p, const InstDev& I, const Lds<T>& l,
                                               const OS* own, const XS* xin, int a, int i, int j,
-                                              double fc) {
+                                              double fc, const int32_t* rowp, const int2* ent) {
   const int d = I.d, d2 = I.d2;
   cpx<T> c = mk<T>(0, 0);
   for (int k = 0; k < d; ++k) {
@@ -261,8 +262,7 @@ __device__ __forceinline__ cpx<T> element_rhs(const StageParams& p, const InstDe
   }
   cpx<T> r = mul_mi(c);
   cpx<T> damp = mk<T>(0, 0);
-  const int r0 = p.rowptr[I.rowptr_off + a], r1 = p.rowptr[I.rowptr_off + a + 1];
-  const int2* ent = p.ent + I.ent_off;
+  const int r0 = rowp[a], r1 = rowp[a + 1];
   const bool dense = (I.flags & F_DENSE) != 0;
   for (int t = r0; t < r1; ++t) {
     const int2 en = ent[t];
@@ -556,7 +556,7 @@ __global__ __launch_bounds__(kThreads) void heom_stage_kernel(StageParams p) {
       if (act) {
         for (int e = le; e < d2; e += G) {
           const int i = e / d, j = e - (e / d) * d;
-          const cpx<T> r = element_rhs<T>(p, I, l, own, xi, a, i, j, fc);
+          const cpx<T> r = element_rhs<T>(p, I, l, own, xi, a, i, j, fc, p.rowptr + I.rowptr_off, p.ent + I.ent_off);
           const int64_t g = I.off + (int64_t)a * d2 + e;
           const bool save = STAGE == 3 && a == 0 && p.traj && p.save_slot >= 0;
           const int64_t tb = p.save_slot * p.traj_stride + I.traj_off;
@@ -1398,75 +1398,189 @@ __global__ __launch_bounds__(kThreads) void heom_resident_kernel(StageParams p) 
   const Lds<T> l = carve<T>(smem, p);
   const InstDev I = p.insts[p.resident[blockIdx.x]];
   const int n = I.nmax * I.d2, d2 = I.d2, d = I.d;
-  cpx<double>* Y = (cpx<double>*)l.rest;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const double dt = p.dt;
+  // ---- LDS: base operators (FP64), the instance's neighbour records, then the state and stage buffers
+  const bool dense = (I.flags & F_DENSE) != 0, sd = (I.flags & F_SDIP) != 0, cd = (I.flags & F_CDIP) != 0;
+  const int nq = dense ? 0 : I.ncoup * d;
+  const int nent = p.rowptr[I.rowptr_off + I.nmax];
+  double2* Hb = (double2*)l.rest;  // H, then the system dipole
+  double2* Qb = Hb + 2 * d2;       // coupling diagonals, then the coupling dipoles
+  cpx<T>* sCL = (cpx<T>*)(Qb + 2 * nq);  // per entry: sqrt-scaled coefficient of Q X' (row term)
+  cpx<T>* sCR = sCL + nent;               // per entry: the same of X' Q (column term)
+  cpx<T>* sDamp = sCR + nent;             // per ADO: sum_k n_k gamma_k
+  int2* sRec = (int2*)(sDamp + I.nmax);   // per entry: neighbour ADO, offset of its coupling (m d, or m)
+  int32_t* sRow = (int32_t*)(sRec + nent);
+  cpx<double>* Y = (cpx<double>*)(((uintptr_t)(sRow + I.nmax + 1) + 15) & ~(uintptr_t)15);
   cpx<double>* Yn = Y + n;
   cpx<T>* X0 = (cpx<T>*)(Yn + n);
   cpx<T>* X1 = X0 + n;
-  const int tid = threadIdx.x;
-  const double dt = p.dt;
-  for (int e = tid; e < n; e += kThreads) {
+  const double2* mat = p.mat;
+  for (int e = tid; e < d2; e += nth) {
+    Hb[e] = mat[I.h_off + e];
+    Hb[d2 + e] = sd ? mat[I.h_off + d2 + e] : make_double2(0, 0);
+  }
+  for (int e = tid; e < nq; e += nth) {
+    Qb[e] = mat[I.q_off + e];
+    Qb[nq + e] = cd ? mat[I.q_off + nq + e] : make_double2(0, 0);
+  }
+  for (int e = tid; e <= I.nmax; e += nth) sRow[e] = p.rowptr[I.rowptr_off + e];
+  // neighbour records: the coefficient, sqrt(n) and coupling of every entry resolved once
+  for (int t = tid; t < nent; t += nth) {
+    const int2 en = p.ent[I.ent_off + t];
+    const int k = en.y & 0xffff, nf = (en.y >> 16) & 0x7f;
+    const bool plus = en.y < 0;
+    const double2* C = p.coef + I.coef_off + 5 * k;
+    const double sq = sqrt((double)nf);
+    const double2 cl = plus ? C[3] : C[1], cr = plus ? C[4] : C[2];
+    sCL[t] = mk<T>((T)(cl.x * sq), (T)(cl.y * sq));
+    sCR[t] = mk<T>((T)(cr.x * sq), (T)(cr.y * sq));
+    const int m = p.mode[I.mode_off + k];
+    sRec[t] = make_int2(en.x, dense ? m : m * d);
+  }
+  __syncthreads();
+  for (int a = tid; a < I.nmax; a += nth) {
+    double gx = 0.0, gy = 0.0;
+    for (int t = sRow[a]; t < sRow[a + 1]; ++t) {
+      const int2 en = p.ent[I.ent_off + t];
+      if (en.y >= 0) {  // minus entries list every k with n_k > 0, with nf = n_k
+        const double2 g = p.coef[I.coef_off + 5 * (en.y & 0xffff)];
+        const double nf = (double)((en.y >> 16) & 0x7f);
+        gx += g.x * nf;
+        gy += g.y * nf;
+      }
+    }
+    sDamp[a] = mk<T>((T)gx, (T)gy);
+  }
+  for (int e = tid; e < n; e += nth) {
     const int a = e / d2, r = e - a * d2, i = r / d;
     const double2 v = ld_state(p.y, state_ref<PACK>(I, a, i, r - i * d));
     Y[e] = mk(v.x, v.y);
   }
+  // H(t) = H + f_s(t) mu and Q(t) = Q + f_c(t) C for field values f_s, f_c, from the LDS copies
+  auto set_ops = [&](double fs, double fc) {
+    for (int e = tid; e < d2; e += nth) {
+      const double2 h = Hb[e], m = Hb[d2 + e];
+      l.H[e] = mk<T>((T)(h.x + m.x * fs), (T)(h.y + m.y * fs));
+    }
+    for (int e = tid; e < nq; e += nth) {
+      const double2 q = Qb[e], c = Qb[nq + e];
+      l.Q[e] = mk<T>((T)(q.x + c.x * fc), (T)(q.y + c.y * fc));
+    }
+  };
+  // right-hand side of element (i, j) of ADO a from input xs (ADO a's matrix at own)
+  auto rhs = [&](const auto* own, const auto* xs, int a, int i, int j, double fc) {
+    cpx<T> c = mk<T>(0, 0);
+    for (int k = 0; k < d; ++k) {
+      cmac(c, l.H[i * d + k], ldc<T>(own, k * d + j));
+      cmsub(c, ldc<T>(own, i * d + k), l.H[k * d + j]);
+    }
+    cpx<T> r = mul_mi(c);
+    cmsub(r, sDamp[a], ldc<T>(own, i * d + j));
+    const int r1 = sRow[a + 1];
+    if (!dense) {
+#pragma unroll 4
+      for (int t = sRow[a]; t < r1; ++t) {
+        const int2 rc = sRec[t];
+        const cpx<T> coef = sCL[t] * l.Q[rc.y + i] + sCR[t] * l.Q[rc.y + j];
+        cmac(r, coef, ldc<T>(xs, (int64_t)rc.x * d2 + i * d + j));
+      }
+    } else {
+      for (int t = sRow[a]; t < r1; ++t) {
+        const int2 rc = sRec[t];
+        // dense coupling: cL (Q X')_ij + cR (X' Q)_ij, Q (+ fc C) read from the FP64 matrix pool
+        const double2* Qm = mat + I.q_off + (int64_t)rc.y * d2;
+        const double2* Cm = cd ? Qm + (int64_t)I.ncoup * d2 : nullptr;
+        const int64_t nb = (int64_t)rc.x * d2;
+        cpx<T> s1 = mk<T>(0, 0), s2 = mk<T>(0, 0);
+        for (int q = 0; q < d; ++q) {
+          double2 qa = Qm[i * d + q], qb = Qm[q * d + j];
+          if (Cm) {
+            const double2 ca = Cm[i * d + q], cb = Cm[q * d + j];
+            qa.x += ca.x * fc;
+            qa.y += ca.y * fc;
+            qb.x += cb.x * fc;
+            qb.y += cb.y * fc;
+          }
+          cmac(s1, mk<T>((T)qa.x, (T)qa.y), ldc<T>(xs, nb + q * d + j));
+          cmac(s2, ldc<T>(xs, nb + i * d + q), mk<T>((T)qb.x, (T)qb.y));
+        }
+        r = r + sCL[t] * s1 + sCR[t] * s2;
+      }
+    }
+    return r;
+  };
+  const bool driven = p.field != nullptr;
+  __syncthreads();
+  if (!driven) set_ops(0.0, 0.0);
+  // the field values of a step (f_s, f_c at t, t + dt/2, t + dt), read one step ahead
+  double fnext[6] = {0, 0, 0, 0, 0, 0};
+  if (driven && p.nsteps > 0)
+    for (int c = 0; c < 6; ++c) fnext[c] = p.field[6 * p.step + c];
   for (int64_t s = 0; s < p.nsteps; ++s) {
-    const int64_t step = p.step + s;
+    double f[6];
+    for (int c = 0; c < 6; ++c) f[c] = fnext[c];
+    if (driven && s + 1 < p.nsteps)
+      for (int c = 0; c < 6; ++c) fnext[c] = p.field[6 * (p.step + s + 1) + c];
 #pragma unroll 1
     for (int stage = 0; stage < 4; ++stage) {
-      __syncthreads();
-      load_instance<T>(p, I, l, stage_col(stage), step);
-      double fc = 0.0;
-      if (p.field) fc = p.field[6 * step + 3 + stage_col(stage)];
-      __syncthreads();
+      const int col = stage_col(stage);
+      if (driven && stage != 2) {  // stages 1 and 2 share the time t + dt/2
+        __syncthreads();            // every read of the previous operators is done
+        set_ops(f[col], f[3 + col]);
+      }
+      __syncthreads();  // operators and the previous stage's output are in place
+      const double fc = f[3 + col];
       const cpx<T>* xs = stage == 1 ? X0 : (stage == 2 ? X1 : X0);
       cpx<T>* xd = stage == 1 ? X1 : X0;
-      for (int e = tid; e < n; e += kThreads) {
+      for (int e = tid; e < n; e += nth) {
         const int a = e / d2, r = e - a * d2;
         const int i = r / d, j = r - (r / d) * d;
+        // Hermitian (packed) plans: every ADO stays Hermitian, so only upper elements are computed
+        // and the lower ones are written as their conjugates
+        if (PACK && i > j) continue;
+        const int eT = a * d2 + j * d + i;
         cpx<T> rr;
         if (stage == 0) {
           // RHS input is y (double), converted to T as it is read
-          rr = element_rhs<T>(p, I, l, Y + a * d2, Y, a, i, j, fc);
+          rr = rhs(Y + a * d2, Y, a, i, j, fc);
         } else {
-          rr = element_rhs<T>(p, I, l, xs + a * d2, xs, a, i, j, fc);
+          rr = rhs(xs + a * d2, xs, a, i, j, fc);
         }
         const double rx = (double)rr.x, ry = (double)rr.y;
         const cpx<double> yv = Y[e];
         if (stage == 0) {
           Yn[e] = mk(yv.x + rx * (dt / 6), yv.y + ry * (dt / 6));
-          xd[e] = mk<T>((T)(yv.x + rx * (dt / 2)), (T)(yv.y + ry * (dt / 2)));
+          const cpx<T> o = mk<T>((T)(yv.x + rx * (dt / 2)), (T)(yv.y + ry * (dt / 2)));
+          xd[e] = o;
+          if (PACK && eT != e) xd[eT] = conjg(o);
         } else if (stage < 3) {
           const cpx<double> an = Yn[e];
           Yn[e] = mk(an.x + rx * (dt / 3), an.y + ry * (dt / 3));
           const double h = stage == 1 ? dt / 2 : dt;
-          xd[e] = mk<T>((T)(yv.x + rx * h), (T)(yv.y + ry * h));
+          const cpx<T> o = mk<T>((T)(yv.x + rx * h), (T)(yv.y + ry * h));
+          xd[e] = o;
+          if (PACK && eT != e) xd[eT] = conjg(o);
         }
         if (stage == 3) {
           const cpx<double> an = Yn[e];
           // Y is not read by other threads in stage 3 (its input is X0), so it can be updated in place
-          Y[e] = mk(an.x + rx * (dt / 6), an.y + ry * (dt / 6));
+          const cpx<double> yn = mk(an.x + rx * (dt / 6), an.y + ry * (dt / 6));
+          Y[e] = yn;
+          if (PACK && eT != e) Y[eT] = conjg(yn);
         }
       }
     }
     __syncthreads();
-    if (PACK) {
-      // packed (Hermitian) state: the lower triangle is the conjugate of the upper one after every
-      // step, as a reload of the packed y would make it
-      for (int e = tid; e < n; e += kThreads) {
-        const int a = e / d2, r = e - a * d2, i = r / d, j = r - i * d;
-        if (i > j) Y[e] = conjg(Y[a * d2 + j * d + i]);
-      }
-      __syncthreads();
-    }
     if (p.traj && ((s + 1) % p.save_every) == 0) {
       const int64_t slot = (s + 1) / p.save_every - 1;
-      for (int e = tid; e < d2; e += kThreads) {
+      for (int e = tid; e < d2; e += nth) {
         p.traj[slot * p.traj_stride + I.traj_off + e] = make_double2(Y[e].x, Y[e].y);
       }
     }
   }
   __syncthreads();
-  for (int e = tid; e < n; e += kThreads) {
+  for (int e = tid; e < n; e += nth) {
     const int a = e / d2, r = e - a * d2, i = r / d;
     const StateRef sr = state_ref<PACK>(I, a, i, r - i * d);
     if (!sr.lower) p.y[sr.g] = make_double2(Y[e].x, Y[e].y);
@@ -1628,6 +1742,7 @@ struct pyqed_heom_plan {
   double2* d_coefp = nullptr;
   int n_proj = 0;
   std::vector<char> is_res;
+  std::vector<size_t> res_extra;  // per instance: resident_extra_bytes
   int64_t tile_target = 0;
   int blocks_per_cu = 8;
   int variant = 0;  // 0: site-projector kernel; 1: pipelined kernel for the MFMA class (Hermitian, FP32)
@@ -1690,9 +1805,19 @@ size_t pipe_lds_bytes(const pyqed_heom_plan* P, int c) {
   return proj_block_bytes(q.d2pad, q.Kmax, q.Lmax, sizeof(float), false) + (kThreads / 64) * pipe_wave_bytes(q.d2pad, q.rowbuf);
 }
 
-size_t resident_lds_bytes(const pyqed_heom_plan* P, int64_t n_el, size_t tsize) {
+// LDS of the resident kernel for an instance of n_el elements whose base operators and tables take
+// `extra` bytes (resident_extra_bytes): header, base operators, tables, state and stage buffers.
+size_t resident_lds_bytes(const pyqed_heom_plan* P, int64_t n_el, size_t extra, size_t tsize) {
   size_t h = lds_header_bytes(P->d2max, P->qmax, P->Kmax, P->Lmax, tsize);
-  return h + (size_t)n_el * (32 + 2 * 2 * tsize);
+  return h + extra + 16 + (size_t)n_el * (32 + 2 * 2 * tsize);
+}
+
+// Base operators (H and system dipole; coupling diagonals and coupling dipoles, FP64 complex), the
+// neighbour records (two complex coefficients and an int2 per entry), the damping per ADO and the row
+// pointers of one instance, as the resident kernel lays them out (sized for FP64 records).
+size_t resident_extra_bytes(int d, int ncoup, bool dense, int nmax, int64_t nent) {
+  const size_t nq = dense ? 0 : (size_t)ncoup * d;
+  return 16 * (2 * (size_t)d * d + 2 * nq) + (2 * 16 + 8) * (size_t)nent + 16 * (size_t)nmax + 4 * (size_t)(nmax + 1);
 }
 
 // The work queue and, per instance, the range [first, last) of its items: tile() lists the items of one
@@ -2011,6 +2136,7 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     }
     P->inst.push_back(I);
     P->nmax.push_back(t.nmax);
+    P->res_extra.push_back(resident_extra_bytes(s.d, s.ncoup, dense, t.nmax, (int64_t)t.ent.size()));
     P->inst_K.push_back(s.K);
     P->inst_L.push_back(s.L);
     off += (int64_t)t.nmax * I.d2;
@@ -2042,7 +2168,7 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
   if (!(flags & PYQED_PLAN_NO_RESIDENT)) {
     for (int i = 0; i < n_inst; ++i) {
       const int64_t n = (int64_t)P->inst[i].nmax * P->inst[i].d2;
-      if (resident_lds_bytes(P, n, sizeof(double)) <= (size_t)P->max_lds) is_res[i] = 1;
+      if (resident_lds_bytes(P, n, P->res_extra[i], sizeof(double)) <= (size_t)P->max_lds) is_res[i] = 1;
     }
   }
   // ---- work items: tiles of ADOs, streamed instances first
@@ -2253,7 +2379,7 @@ StageParams base_params(const pyqed_heom_plan* P) {
 }
 
 template <typename Kern>
-int launch(Kern k, int grid, size_t lds, hipStream_t s, const StageParams& p) {
+int launch(Kern k, int grid, size_t lds, hipStream_t s, const StageParams& p, int threads = kThreads) {
   // raise the kernel's dynamic-LDS limit only when a larger size is first needed, so launches that
   // are being captured into a graph make no attribute calls
   static std::mutex mu;
@@ -2266,7 +2392,7 @@ int launch(Kern k, int grid, size_t lds, hipStream_t s, const StageParams& p) {
       cur = lds;
     }
   }
-  hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), lds, s, p);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(threads), lds, s, p);
   PQA_HIP(hipGetLastError());
   return PYQED_OK;
 }
@@ -2429,9 +2555,11 @@ int run_typed(pyqed_heom_plan* P, double2* y, char* ws, const double* field, int
     r.step = step0;
     r.nsteps = nsteps;
     r.save_every = save_every;
-    int64_t nres_max = 0;
-    for (int i : P->resident) nres_max = std::max<int64_t>(nres_max, (int64_t)P->inst[i].nmax * P->inst[i].d2);
-    const size_t lds = resident_lds_bytes(P, nres_max, sizeof(T));
+    size_t lds = 0;
+    for (int i : P->resident)
+      lds = std::max(lds, resident_lds_bytes(P, (int64_t)P->inst[i].nmax * P->inst[i].d2, P->res_extra[i], sizeof(T)));
+    // four waves per instance: an element's right-hand side is a chain of dependent LDS reads, so
+    // even small instances run faster with one element per lane than with one wave and cheaper barriers
     int rc = P->packed ? launch(heom_resident_kernel<T, true>, (int)P->resident.size(), lds, P->side, r)
                        : launch(heom_resident_kernel<T, false>, (int)P->resident.size(), lds, P->side, r);
     if (rc) return rc;
