@@ -778,6 +778,113 @@ def bench_krylov(args, torch):
     print(json.dumps(res), flush=True)
 
 
+HEOM_ADJ_METRIC = "HEOM gradients/s (streamed FP64 adjoint, 7-site exciton depth 4, 1000 steps), 1 GPU"
+
+
+def heom_adjoint_problem(depth=4, nseg=50, sps=20, ckpt_every=50):
+    """7-site exciton (K = 14, Drude baths) with H(theta) = H + sum_j theta_j |j><j| over all sites and a
+    transition dipole (nearest-neighbour hopping operator) driven by a piecewise-constant field of
+    nseg segments of sps steps (dt = 1 fs); J = population of site 2 at T."""
+    from pyqed_amd import workloads as W
+    from pyqed_amd.heom.adjoint import HeomControl
+    from pyqed_amd.heom.bath import Bath
+    w = W.exciton_c2()
+    d = w["H"].shape[0]
+    b = Bath.drude(w["lam"], w["gam"], w["beta"], 1, nbath=d)
+    P = [np.diag(np.eye(d)[j]).astype(complex) for j in range(d)]
+    mu = np.zeros((d, d), complex)
+    for j in range(d - 1):
+        mu[j, j + 1] = mu[j + 1, j] = 1.0
+    prob = HeomControl(w["H"], P, mu, w["Q"], (b.etal, b.etar, b.etaa, b.expn, b.mode), depth, w["rho0"], w["dt"],
+                       sps, nseg, ckpt_every=ckpt_every)
+    theta = np.zeros(d)
+    amps = np.random.default_rng(7).normal(0.0, 2e-4, nseg)
+    bar = np.zeros((d, d), complex)
+    bar[2, 2] = 1.0
+    return prob, theta, amps, bar
+
+
+def bench_heom_adjoint(args, torch):
+    """One gradient of a final population with respect to 7 site energies and 50 field amplitudes through
+    1000 RK4 steps of a 3060-ADO hierarchy: forward run with checkpoints, then the backward sweep (per
+    step 3 forward and 4 adjoint right-hand sides on the FP64 stage kernel and one contraction)."""
+    from pyqed_amd import _lib
+    from pyqed_amd.heom.adjoint import HeomAdjoint, piecewise_field
+    prob, theta, amps, bar = heom_adjoint_problem()
+    prob.gradient(theta, amps, bar)  # warm-up (plan build, code load)
+    torch.cuda.synchronize()
+    reps = max(1, args.steps if args.steps else 1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gt, ga, _, r = prob.gradient(theta, amps, bar)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    # the dominant kernel: the FP64 right-hand side (generic stage kernel) on the adjoint plan
+    H = prob.H0 + sum(float(t) * p for t, p in zip(theta, prob.P))
+    inst = dict(H=H, Q=prob.Q, bath=prob.bath, L=prob.Ldepth, rho0=prob.rho0, sdip=prob.mu)
+    ha = HeomAdjoint([inst], prob.dt, prob.nt, ckpt_every=prob.ckpt_every)
+    f = torch.as_tensor(piecewise_field(amps, prob.sps), device=ha.fwd.device)
+    x = torch.randn(ha.fwd.n_elems, dtype=torch.complex128, device=ha.fwd.device)
+    out = torch.empty_like(x)
+    st = torch.cuda.current_stream()
+    for _ in range(3):
+        ha._rhs(ha.adj, x, out, f, 0, 1)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record(st)
+    for _ in range(200):
+        ha._rhs(ha.adj, x, out, f, 0, 1)
+    ev[1].record(st)
+    torch.cuda.synchronize()
+    rhs_ms = ev[0].elapsed_time(ev[1]) / 200
+    M = int(ha.fwd.nmax[0]) * 49
+    ha.close()
+    byts = 32.0 * M  # read X, write L^H X (complex128); neighbour re-reads counted once
+    nt = prob.nt
+    res = {
+        "metric": HEOM_ADJ_METRIC, "value": 1.0 / wall, "unit": "gradients/s", "n_gpus": 1, "steps": reps,
+        "warmup": 1, "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (7-site exciton chain, seeded energies, Drude baths, seeded field)",
+        "config": {"workload": f"7-site exciton HEOM depth 4 (3060 ADOs), {nt} RK4 steps, {prob.nseg} field "
+                               f"segments, gradient w.r.t. 7 site energies + {prob.nseg} amplitudes, checkpoints "
+                               f"every {prob.ckpt_every} steps",
+                   "backward_us_per_step": wall / nt * 1e6},
+        "roofline": {"bound": "hbm", "kernel": "heom_stage_kernel<double, double, double, 4, false> (L^H x, "
+                                               "adjoint plan)", "launch_ms": rhs_ms, "bytes_per_launch": byts,
+                     "achieved": byts / (rhs_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": byts / (rhs_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "note": "state of 2.4 MB: cache-resident and launch-latency bound, not HBM-meaningful"},
+        "gradient_norm": float(np.sqrt(np.sum(gt ** 2) + np.sum(ga ** 2))),
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = heom_adjoint_cpu_baseline(prob, theta, amps)
+    print(json.dumps(res), flush=True)
+
+
+def heom_adjoint_cpu_baseline(prob, theta, amps, budget_steps=20):
+    """FP64 torch autograd of the same propagation (the oracle's restatement) on the host's cores, over
+    the first budget_steps steps (one segment), scaled to the full step count."""
+    import torch
+    from oracle import heom_torch as T
+    model, host = cpu_info()
+    nthr = max(1, min(16, host))
+    torch.set_num_threads(nthr)
+    etal, etar, etaa, expn, mode = prob.bath
+    th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    nseg = max(1, budget_steps // prob.sps)
+    am = torch.tensor(amps[:nseg], dtype=torch.float64, requires_grad=True)
+    t0 = time.perf_counter()
+    rT = T.final_state(th, am, prob.H0, np.array(prob.P), prob.mu, prob.Q, (etal, etar, etaa, expn), mode,
+                       prob.Ldepth, prob.rho0, prob.dt, prob.sps)
+    rT[2, 2].real.backward()
+    el = time.perf_counter() - t0
+    steps = nseg * prob.sps
+    per_grad = el / steps * prob.nt
+    return {"value": 1.0 / per_grad, "unit": "gradients/s", "cores": nthr, "kind": "port", "cpu_model": model,
+            "host_cores": host,
+            "sample": f"oracle.heom_torch (FP64 torch autograd, {nthr} threads): forward and backward through the "
+                      f"first {steps} steps in {el:.1f} s, scaled to {prob.nt} steps"}
+
+
 # Full-length accuracy check per workload: the step count each configuration is quoted at.
 FULL_STEPS = {"c1": 1000, "c2": 1000, "c3": 2000, "c5": 5000}
 
@@ -998,7 +1105,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; 2000 for --workload legacy)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "lindblad", "legacy",
-                                                              "legacy-rk4", "spectral", "krylov"])
+                                                              "legacy-rk4", "spectral", "krylov", "heom-adjoint"])
     ap.add_argument("--n-inst", type=int, default=None, help="instances (default 4096; 65536 for legacy)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--state", default="auto", choices=["auto", "full"],
@@ -1026,7 +1133,7 @@ def main():
     args = ap.parse_args()
     legacy = args.workload in ("legacy", "legacy-rk4")
     if args.steps is None:
-        args.steps = 2000 if legacy else (3 if args.workload in ("c4", "lindblad") else 20)
+        args.steps = 2000 if legacy else (3 if args.workload in ("c4", "lindblad", "heom-adjoint") else 20)
     if args.n_inst is None:
         args.n_inst = 65536 if legacy else 4096
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -1062,6 +1169,9 @@ def main():
         return
     if args.workload == "krylov":
         bench_krylov(args, torch)
+        return
+    if args.workload == "heom-adjoint":
+        bench_heom_adjoint(args, torch)
         return
 
     if args.traffic_from:

@@ -1954,6 +1954,25 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
       destroy_plan(P);
       return fail(PYQED_EINVAL, "instance " + std::to_string(i) + ": missing H, bath or coupling arrays");
     }
+    // non-finite inputs would propagate silently through every ADO: reject them here
+    auto finite = [](const double* v, int64_t n) {
+      for (int64_t e = 0; e < n; ++e)
+        if (!std::isfinite(v[e])) return false;
+      return true;
+    };
+    const int64_t nqv = 2 * (int64_t)s.ncoup * (dense ? s.d * s.d : s.d);
+    const char* bad = !finite(s.H, 2 * (int64_t)s.d * s.d) ? "H"
+                      : (s.sdip && !finite(s.sdip, 2 * (int64_t)s.d * s.d)) ? "sdip"
+                      : !finite(dense ? s.qmat : s.qdiag, nqv) ? "coupling"
+                      : ((dense ? s.cmat : s.cdiag) && !finite(dense ? s.cmat : s.cdiag, nqv)) ? "coupling dipole"
+                      : (s.K > 0 && !finite(s.expn, 2 * (int64_t)s.K)) ? "expn"
+                      : (s.K > 0 && !finite(s.etal, 2 * (int64_t)s.K)) ? "etal"
+                      : (s.K > 0 && !finite(s.etar, 2 * (int64_t)s.K)) ? "etar"
+                      : (s.K > 0 && !finite(s.etaa, 2 * (int64_t)s.K)) ? "etaa" : nullptr;
+    if (bad) {
+      destroy_plan(P);
+      return fail(PYQED_EINVAL, "instance " + std::to_string(i) + ": non-finite value (NaN or Inf) in " + bad);
+    }
     for (int k = 0; k < s.K; ++k) {
       if (s.mode[k] < 0 || s.mode[k] >= s.ncoup) {
         destroy_plan(P);
@@ -2720,20 +2739,36 @@ __global__ __launch_bounds__(kThreads) void heom_adjoint_contract_kernel(AdjPara
   }
 }
 
-// One block per instance: G[c][traj_off + f] = sum over the instance's items of their partials.
+// G[c][traj_off + f] = sum over the instance's items of their partials.  Block (instance, x) covers 16
+// of the instance's 3 d^2 values; its 256 threads are 16 values x 16 item lanes, each lane summing
+// every 16th item, and the 16 lane sums are added in a fixed order (deterministic results).
+constexpr int kRedVals = 16, kRedLanes = kThreads / kRedVals;
+
 __global__ __launch_bounds__(kThreads) void heom_adjoint_reduce_kernel(AdjParams q) {
-  const int i = blockIdx.x;
+  __shared__ double2 red[kRedLanes][kRedVals];
+  const int i = blockIdx.y;
   const InstDev I = q.insts[i];
   const int2 r = q.inst_items[i];
-  for (int e = threadIdx.x; e < 3 * I.d2; e += kThreads) {
-    const int c = e / I.d2, f = e - c * I.d2;
-    double2 sum = make_double2(0.0, 0.0);
-    for (int it = r.x; it < r.y; ++it) {
-      const double2 v = q.part[((int64_t)it * 3 + c) * q.d2max + f];
-      sum.x += v.x;
-      sum.y += v.y;
+  const int v = threadIdx.x % kRedVals, ln = threadIdx.x / kRedVals;
+  const int e = blockIdx.x * kRedVals + v;
+  const bool act = e < 3 * I.d2;
+  const int c = act ? e / I.d2 : 0, f = act ? e - c * I.d2 : 0;
+  double2 sum = make_double2(0.0, 0.0);
+  if (act)
+    for (int it = r.x + ln; it < r.y; it += kRedLanes) {
+      const double2 w = q.part[((int64_t)it * 3 + c) * q.d2max + f];
+      sum.x += w.x;
+      sum.y += w.y;
     }
-    q.G[c * q.traj_stride + I.traj_off + f] = sum;
+  red[ln][v] = sum;
+  __syncthreads();
+  if (ln == 0 && act) {
+    double2 t = red[0][v];
+    for (int k = 1; k < kRedLanes; ++k) {
+      t.x += red[k][v].x;
+      t.y += red[k][v].y;
+    }
+    q.G[c * q.traj_stride + I.traj_off + f] = t;
   }
 }
 
@@ -2778,7 +2813,9 @@ extern "C" int pyqed_heom_adjoint_contract(pyqed_heom_plan* P, const void* const
   const int grid = std::max(1, std::min(q.nitems, P->num_cu * 8));
   hipLaunchKernelGGL(heom_adjoint_contract_kernel, dim3(grid), dim3(kThreads), adjoint_lds_bytes(P), st, q);
   PQA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(heom_adjoint_reduce_kernel, dim3(P->n_inst), dim3(kThreads), 0, st, q);
+  if (P->n_inst > 65535) return fail(PYQED_EINVAL, "heom_adjoint_contract: at most 65535 instances per plan");
+  const unsigned nx = (unsigned)((3 * P->d2max + kRedVals - 1) / kRedVals);
+  hipLaunchKernelGGL(heom_adjoint_reduce_kernel, dim3(nx, P->n_inst), dim3(kThreads), 0, st, q);
   PQA_HIP(hipGetLastError());
   return PYQED_OK;
 }

@@ -120,8 +120,12 @@ class DEOMSolver:
                     bath=(b.etal, b.etar, b.etaa, b.expn, np.asarray(b.mode)))
         return inst
 
-    def run(self, rho0, dt, nt, p1=None, *, precision="fp64", save_every=1, inplace=False):
-        """Propagate nt RK4 steps; returns (t_save, ddos_save) as deom.py:1072-1114."""
+    def run(self, rho0, dt, nt, p1=None, *, precision="fp64", save_every=1, inplace=False, trace_tol=None):
+        """Propagate nt RK4 steps; returns (t_save, ddos_save) as deom.py:1072-1114.
+
+        trace_tol: when given, raise RuntimeError if |Tr rho(t) - Tr rho0| exceeds it at any saved time
+        (every coupling term of the hierarchy leaves the trace of rho unchanged, so drift means a broken
+        run: a non-finite value, or a step far beyond stability)."""
         self.check_()
         self.init_()
         inst = self._instance(rho0)
@@ -147,6 +151,14 @@ class DEOMSolver:
             field = eng.torch.as_tensor(tab, device=eng.device)
         traj = eng.run(dt, nt, precision=precision, field=field, save_every=save_every)[0]
         t_save = np.arange(traj.shape[0], dtype=np.float64) * dt * save_every
+        if trace_tol is not None:
+            tr = np.trace(traj, axis1=1, axis2=2)
+            drift = np.abs(tr - tr[0])
+            bad = ~np.isfinite(drift) | (drift > trace_tol)
+            if np.any(bad):
+                k = int(np.argmax(bad))
+                raise RuntimeError(f"trace drift {drift[k]:.3e} > {trace_tol:.1e} at t = {t_save[k]:.6g} "
+                                   f"(saved step {k})")
         if inplace:
             np.copyto(rho0, traj[-1])
         if p1 is None:

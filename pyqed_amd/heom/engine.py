@@ -48,7 +48,10 @@ def common_eigenbasis(mats, tol=1e-10):
 
 
 def prepare_instance(inst):
-    """Choose the working basis of one instance; returns a dict of complex128 arrays ready for the plan."""
+    """Choose the working basis of one instance; returns a dict of complex128 arrays ready for the plan.
+    Non-finite initial states raise ValueError (the library rejects non-finite operators and baths)."""
+    if not np.all(np.isfinite(np.asarray(inst["rho0"], np.complex128))):
+        raise ValueError("rho0 has a non-finite entry (NaN or Inf)")
     H = np.asarray(inst["H"], np.complex128)
     d = H.shape[0]
     Q = np.asarray(inst["Q"], np.complex128).reshape(-1, d, d)
@@ -231,6 +234,8 @@ class HeomEngine:
         torch = self.torch
         for k, p in enumerate(self.prep):
             a = np.asarray(states[k], np.complex128)
+            if not np.all(np.isfinite(a)):
+                raise ValueError(f"set_state: instance {k} has a non-finite entry (NaN or Inf)")
             U = p["U"]
             if U is not None:
                 a = U.conj().T @ a @ U
@@ -446,5 +451,7 @@ def field_table(nt, dt, fs=None, fc=None, step0=0):
                 v = complex(f(tt))
                 if v.imag != 0.0:
                     raise NotImplementedError("complex-valued pulse functions are not supported")
+                if not np.isfinite(v.real):
+                    raise ValueError(f"pulse function returned {v.real} at t = {tt}")
                 tab[i, col] = v.real
     return tab

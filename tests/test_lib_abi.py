@@ -63,3 +63,45 @@ def test_tables_against_oracle_random_shapes():
         assert (keys == okeys).all() and (plus == op).all() and (minus == om).all()
 
     check()
+
+
+def _instance_struct(d=2, K=2, bad=None):
+    """A valid two-level instance for pyqed_heom_plan_create, optionally with one non-finite array."""
+    from pyqed_amd import _lib
+    arrs = dict(H=np.array([[0.5, 0.1], [0.1, -0.5]], complex), qdiag=np.array([[1.0, -1.0]], complex),
+                expn=np.array([1.0, 6.0], complex), etal=np.array([0.2 - 0.05j, 0.06], complex),
+                etar=np.array([0.2 + 0.05j, 0.06], complex), etaa=np.array([0.2, 0.06], complex),
+                mode=np.zeros(K, np.int32))
+    if bad is not None:
+        name, val = bad
+        arrs[name] = arrs[name].copy()
+        arrs[name].flat[0] = val
+    s = _lib.HeomInstance()
+    s.d, s.K, s.L, s.ncoup = d, K, 3, 1
+    for k, v in arrs.items():
+        setattr(s, k, v.ctypes.data)
+    return s, arrs
+
+
+@pytest.mark.parametrize("name", ["H", "qdiag", "expn", "etal", "etar", "etaa"])
+@pytest.mark.parametrize("val", [np.nan, np.inf, -np.inf])
+def test_plan_rejects_non_finite_inputs(name, val):
+    """NaN / Inf in an operator or a bath coefficient is an error code from plan_create, reported before
+    any device work (no GPU is touched)."""
+    from pyqed_amd import _lib
+    L = _lib.lib()
+    s, keep = _instance_struct(bad=(name, val))
+    plan = C.c_void_p()
+    rc = L.pyqed_heom_plan_create(C.byref(s), 1, 0, C.byref(plan))
+    assert rc == 1 and not plan.value
+    assert b"non-finite" in L.pyqed_last_error()
+
+
+def test_host_checks_reject_non_finite_rho0_and_fields():
+    from pyqed_amd.heom.engine import field_table, prepare_instance
+    inst = dict(H=np.eye(2), Q=np.array([np.diag([1.0, -1.0])]), L=2, rho0=np.array([[np.nan, 0], [0, 0]]),
+                bath=(np.ones(1), np.ones(1), np.ones(1), np.ones(1), np.zeros(1, int)))
+    with pytest.raises(ValueError, match="non-finite"):
+        prepare_instance(inst)
+    with pytest.raises(ValueError, match="pulse"):
+        field_table(3, 0.1, lambda t: np.inf if t > 0.15 else 0.0)
