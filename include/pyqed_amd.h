@@ -168,9 +168,11 @@ int pyqed_heom_plan_stats(const pyqed_heom_plan* plan, int64_t* stats4);
  * set_tiling synchronises the device before replacing the queue. */
 int pyqed_heom_plan_set_tiling(pyqed_heom_plan* plan, int64_t target_elems, int32_t blocks_per_cu);
 int pyqed_heom_plan_get_tiling(const pyqed_heom_plan* plan, int64_t* target_elems, int32_t* blocks_per_cu);
-/* Kernel variant of the MFMA class for Hermitian FP32 runs: 0 = one ADO per
- * wavefront at a time, 1 = two-deep software pipeline.  Chosen by the
- * autotuner; results are identical. */
+/* Kernel variants, chosen by the autotuner; results are identical.  Bit 0: the MFMA class (12 <= d <= 16)
+ * of Hermitian full-matrix FP32 runs uses the two-deep software-pipelined kernel instead of one ADO per
+ * wavefront at a time.  Bit 1: the wide class (17 <= d <= 32) of packed FP32 runs uses the
+ * block-cooperative kernel (one ADO per block, one 16 x 16 MFMA tile per wavefront) instead of one ADO
+ * per wavefront on 2 x 2 tiles.  variant in 0..3. */
 int pyqed_heom_plan_set_variant(pyqed_heom_plan* plan, int32_t variant);
 /* Streamed work per kernel class (0 generic, 1 site-projector with VALU
  * commutator, 2 site-projector with MFMA commutator on one 16 x 16 tile (d <= 16),

@@ -617,6 +617,12 @@ constexpr int kChunk = PQA_CHUNK;  // neighbour loads per lane issued together
 // the neighbour-row requests (both round trips overlap the commutator)
 #define PQA_RK4_HOIST 1
 #endif
+#ifndef PQA_PACK_HOIST
+// packed MFMA forms: 0 request the RK4 operands of an ADO's upper elements once its neighbour rows
+// have arrived; 1 request them right behind the first chunk of neighbour rows (they then arrive during
+// the commutator, at the cost of the registers that hold them meanwhile)
+#define PQA_PACK_HOIST 0
+#endif
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -740,7 +746,7 @@ void heom_proj_kernel(StageParams p) {
   constexpr bool kPM = kDelta && PACK;  // packed mixed run: two FP32 registers (rk4_upper_m)
   static_assert(NT == 1 || (NT == 2 && MFMA), "2 x 2 tiles are an MFMA form");
 #ifndef PQA_WIDE_PF
-#define PQA_WIDE_PF 4  // own-matrix elements per lane the wide class prefetches (the rest load in place)
+#define PQA_WIDE_PF 9  // own-matrix elements per lane the wide wave form prefetches (the rest load in place)
 #endif
   constexpr int kPF = NT == 1 ? 4 : PQA_WIDE_PF;  // own-matrix elements per lane prefetched (64 kPF)
   constexpr int kQP = NT == 1 ? 3 : 9;  // packed upper elements per lane, d(d+1)/2 <= 64 kQP
@@ -870,6 +876,20 @@ void heom_proj_kernel(StageParams p) {
       cpx<T> acc[4 * NT * NT];  // MFMA layout: element (16 ti + 4 gk + r, 16 tj + gi) at (ti NT + tj) 4 + r
       Pre<kDelta> pr[NT == 1 ? 4 : kQP];
       PreM pm[NT == 1 ? 4 : kQP];
+      // packed mode: the RK4 operands of this lane's upper elements (packed indices lane + 64q, at most
+      // 64 kQP of them), read in packed order, either here or once the neighbour rows are in
+      auto request_packed = [&]() {
+#pragma unroll
+        for (int q = 0; q < kQP; ++q) {
+          const int pk = lane + 64 * q;
+          if (pk < ((d * (d + 1)) >> 1)) {
+            const int64_t g = I.poff + (int64_t)a * I.dp + pk;
+            if constexpr (kPM) pm[q] = rk4_prefetch_m<STAGE>(p, g);
+            else pr[q] = rk4_prefetch<STAGE, kDelta>(p, StateRef{g, false});
+          }
+        }
+      };
+      if (MFMA && PACK && !(kPM && STAGE == 0) && PQA_PACK_HOIST == 1) request_packed();
       if (MFMA && NT == 1 && !PACK && PQA_RK4_HOIST == 2) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -948,17 +968,7 @@ void heom_proj_kernel(StageParams p) {
       // RK4 operands of the lane's four elements, requested together (one round trip instead of
       // one per element); the gather below overlaps their latency
       if (MFMA && PACK && !(kPM && STAGE == 0)) {
-        // packed mode: RK4 operands of this lane's upper elements, packed indices lane + 64q (at most
-        // 64 kQP of them), read in packed order
-#pragma unroll
-        for (int q = 0; q < kQP; ++q) {
-          const int pk = lane + 64 * q;
-          if (pk < ((d * (d + 1)) >> 1)) {
-            const int64_t g = I.poff + (int64_t)a * I.dp + pk;
-            if constexpr (kPM) pm[q] = rk4_prefetch_m<STAGE>(p, g);
-            else pr[q] = rk4_prefetch<STAGE, kDelta>(p, StateRef{g, false});
-          }
-        }
+        if (PQA_PACK_HOIST == 0) request_packed();
       } else if (MFMA && NT == 1 && PQA_RK4_HOIST == 1) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -1156,6 +1166,278 @@ void heom_proj_kernel(StageParams p) {
         }
       }
       wave_sync();
+    }
+  }
+}
+
+// ------------------------------------------------------------------ block-cooperative wide kernel
+// The wide class (17 <= d <= 32, site projectors) in packed Hermitian mixed runs, one ADO per block at
+// a time: the four waves share its neighbour-row, own-matrix and RK4-operand requests (one round trip
+// for all of them, a few values per lane), and each wave forms one 16 x 16 tile of the 2 x 2 MFMA
+// commutator.  Wave 0 builds the site-ordered neighbour list (as the projector kernel does per wave).
+#ifndef PQA_WIDE_ROWS
+#define PQA_WIDE_ROWS 6  // neighbour-row values per lane requested together (a further pass beyond)
+#endif
+#ifndef PQA_WIDE_BLOCK_WAVES
+#define PQA_WIDE_BLOCK_WAVES 3  // minimum waves per SIMD the block kernel is compiled for (no spills)
+#endif
+
+// Per-block ADO buffers of the wide block kernel: own matrix / output tile (d2pad), row buffer, and two
+// copies (this ADO's and the next one's) of the neighbour list with its coefficients, site-group starts
+// and damping.
+constexpr int kWideList = 2 * 8 * kMaxEnt + 2 * 4 * kMaxEnt + 4 * 36 + 16;  // bytes of one list copy
+__host__ __device__ inline size_t wide_ado_bytes(int d2pad, int rowbuf) {
+  size_t b = 8 * ((size_t)d2pad + rowbuf) + 2 * (size_t)kWideList;
+  return (b + 15) & ~size_t(15);
+}
+
+// One neighbour-list copy of the wide block kernel.
+struct WideList {
+  cpx<float>* cl;   // per entry: coefficient of the row term
+  cpx<float>* cr;   // per entry: coefficient of the column term
+  int32_t* nb;      // per entry: neighbour ADO
+  int32_t* ste;     // per entry: site
+  int32_t* gs;      // site-group starts (d + 1 <= 33); gs[35]: entries of the ADO
+  cpx<float>* damp; // sum_k n_k gamma_k
+};
+
+__device__ __forceinline__ WideList wide_list(char* b) {
+  WideList w;
+  w.cl = (cpx<float>*)b;
+  w.cr = w.cl + kMaxEnt;
+  w.nb = (int32_t*)(w.cr + kMaxEnt);
+  w.ste = w.nb + kMaxEnt;
+  w.gs = w.ste + kMaxEnt;
+  w.damp = (cpx<float>*)(w.gs + 36);
+  return w;
+}
+
+// Wave 0 of the block: the neighbour list of one ADO (entry en in lane < nent) in site order, with its
+// coefficients and damping, into list copy L.
+__device__ __forceinline__ void wide_build_list(const WideList& L, int2 en, int nent, int lane, int d, bool sorted,
+                                                const cpx<float>* sC, const float* sS, const int32_t* sSite) {
+  const int k = en.y & 0xffff, nf = (en.y >> 16) & 0x7f;
+  const bool plus = en.y < 0;
+  const int my_site = lane < nent ? sSite[k] : 0x7fffffff;
+  int gsv;
+  const int pos = site_order(my_site, d, lane, sorted, gsv);
+  if (lane <= d) L.gs[lane] = gsv;
+  cpx<float> dp = mk<float>(0, 0);
+  if (lane < nent) {
+    const cpx<float>* C = sC + 5 * k;
+    const float sq = sS[nf];
+    L.nb[pos] = en.x;
+    L.ste[pos] = my_site;
+    L.cl[pos] = (plus ? C[3] : C[1]) * sq;
+    L.cr[pos] = (plus ? C[4] : C[2]) * sq;
+    if (!plus) dp = C[0] * (float)nf;
+  }
+  const cpx<float> damp = wave_sum(dp);
+  if (lane == 0) {
+    L.damp[0] = damp;
+    L.gs[35] = nent;
+  }
+}
+
+// The ADO loop is software-pipelined: while ADO a computes, wave 0 builds the neighbour list of a + 1
+// in the other list copy and every thread holds a + 1's own matrix in registers, so an ADO waits only
+// for its neighbour rows and RK4 operands, requested together.
+template <int STAGE>
+__global__ __launch_bounds__(kThreads, PQA_WIDE_BLOCK_WAVES) void heom_wide_kernel(StageParams p) {
+  using T = float;
+  extern __shared__ __align__(16) char smem[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  cpx<T>* sH = (cpx<T>*)smem;
+  cpx<T>* sC = sH + p.d2pad;
+  T* sS = (T*)(sC + 5 * p.Kmax);
+  int32_t* sSite = (int32_t*)(sS + p.Lmax + 2);
+  int32_t* sIJ = sSite + p.Kmax;
+  cpx<T>* X = (cpx<T>*)(smem + proj_block_bytes(p.d2pad, p.Kmax, p.Lmax, sizeof(T), true));
+  cpx<T>* R = X + p.d2pad;  // neighbour rows in site order, row t at t (d + 1)
+  char* lists = (char*)(R + p.rowbuf);  // two list copies, kWideList bytes each
+  const cpx_store<T>* xin = (const cpx_store<T>*)p.xin;
+  cpx_store<T>* xout = (cpx_store<T>*)p.xout;
+  const int col = stage_col(STAGE);
+  const int ti = wv >> 1, tj = wv & 1, gi = lane & 15, gk = lane >> 4;
+  constexpr int kOwn = (32 * 32 + kThreads - 1) / kThreads;  // own-matrix elements per thread
+  int it0, itstep, itend;
+  item_range(p, it0, itend, itstep);
+  for (int it = it0; it < itend; it += itstep) {
+    const WorkItem wi = ro_load(p.items + it);
+    const InstDev I = ro_load(p.insts + wi.inst);
+    const int d = I.d, d2 = I.d2, rs = d + 1, np = (d * (d + 1)) >> 1;
+    const uint32_t dmag = ((1u << 20) + d - 1) / d;
+    const bool sorted = (I.flags & F_SITESORT) != 0;
+    const int a0 = wi.ado0, aend = wi.ado0 + wi.nado;
+    const cpx_store<T>* xi = xin + I.off;
+    __syncthreads();  // the previous item's buffers are free
+    proj_header<T>(p, I, col, dmag, sH, sC, sSite, sS, sIJ);
+    // prologue: own matrix of a0 in registers; neighbour list of a0 in list copy 0
+    cpx_store<T> xo[kOwn];
+#pragma unroll
+    for (int q = 0; q < kOwn; ++q) {
+      const int e = tid + kThreads * q;
+      if (e < d2) xo[q] = xi[(int64_t)a0 * d2 + e];
+    }
+    if (wv == 0) {
+      const int r0 = ro_int(p.rowptr, I.rowptr_off + a0);
+      const int nent = ro_int(p.rowptr, I.rowptr_off + a0 + 1) - r0;
+      int2 en = make_int2(0, 0);
+      if (lane < nent) en = p.ent[I.ent_off + r0 + lane];
+      __syncthreads();  // header in place (sSite, sC, sS)
+      wide_build_list(wide_list(lists), en, nent, lane, d, sorted, sC, sS, sSite);
+    } else {
+      __syncthreads();
+    }
+    for (int a = a0; a < aend; ++a) {
+      const int par = (a - a0) & 1;
+      const WideList L = wide_list(lists + par * kWideList);
+      const WideList Ln = wide_list(lists + (par ^ 1) * kWideList);
+      const int64_t base = (int64_t)a * d2;
+      const bool has_next = a + 1 < aend;
+      __syncthreads();  // list(a) is built; the previous ADO's tile and rows are no longer read
+      // ---- own matrix of a from registers into LDS, then a + 1's requested; wave 0 requests list(a + 1)
+#pragma unroll
+      for (int q = 0; q < kOwn; ++q) {
+        const int e = tid + kThreads * q;
+        if (e < d2) X[e + (int)(((uint32_t)e * dmag) >> 20)] = cvt<T>(mk(xo[q].x, xo[q].y));
+      }
+      int2 en_n = make_int2(0, 0);
+      int nent_n = 0;
+      if (wv == 0 && has_next) {
+        const int r0 = ro_int(p.rowptr, I.rowptr_off + a + 1);
+        nent_n = ro_int(p.rowptr, I.rowptr_off + a + 2) - r0;
+        if (lane < nent_n) en_n = p.ent[I.ent_off + r0 + lane];
+      }
+      const int nval = L.gs[35] * d;
+      // ---- neighbour rows and this thread's RK4 operands (one round trip)
+      cpx<T> v[PQA_WIDE_ROWS];
+#pragma unroll
+      for (int q = 0; q < PQA_WIDE_ROWS; ++q) {
+        const int idx = tid + kThreads * q;
+        if (idx < nval) {
+          const int t = (int)(((uint32_t)idx * dmag) >> 20), jj = idx - t * d;
+          v[q] = ldc<T>(xi, (int64_t)L.nb[t] * d2 + L.ste[t] * d + jj);
+        }
+      }
+      PreM pm[3];
+      if (STAGE != 0) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const int pk = tid + kThreads * q;
+          if (pk < np) pm[q] = rk4_prefetch_m<STAGE>(p, I.poff + (int64_t)a * I.dp + pk);
+        }
+      }
+      if (has_next) {
+#pragma unroll
+        for (int q = 0; q < kOwn; ++q) {
+          const int e = tid + kThreads * q;
+          if (e < d2) xo[q] = xi[base + d2 + e];
+        }
+      }
+      __syncthreads();  // own matrix of a in place
+      // ---- commutator -i[H, X] on MFMA: wave (ti, tj) forms tile (ti, tj) of the 2 x 2 grid
+      cpx<T> acc[4];
+      {
+        floatx4 aR = {0.f, 0.f, 0.f, 0.f}, aI = {0.f, 0.f, 0.f, 0.f};
+        const int ga = 16 * ti + gi, gb = 16 * tj + gi;
+#pragma unroll 1
+        for (int kc = 0; kc < 8; ++kc) {
+          const int kk = 4 * kc + gk;
+          const bool oka = ga < d && kk < d, okb = gb < d && kk < d;
+          const cpx<T> xa = oka ? X[ga * rs + kk] : mk<T>(0, 0);   // A-layout X[ga][kk]
+          const cpx<T> ha = oka ? sH[ga * rs + kk] : mk<T>(0, 0);  // A-layout H[ga][kk]
+          const cpx<T> xb = okb ? X[kk * rs + gb] : mk<T>(0, 0);   // B-layout X[kk][gb]
+          const cpx<T> hb = okb ? sH[kk * rs + gb] : mk<T>(0, 0);  // B-layout H[kk][gb]
+          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(ha.x, xb.x, aR, 0, 0, 0);
+          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(-ha.y, xb.y, aR, 0, 0, 0);
+          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(ha.x, xb.y, aI, 0, 0, 0);
+          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(ha.y, xb.x, aI, 0, 0, 0);
+          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.x, -hb.x, aR, 0, 0, 0);
+          aR = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.y, hb.y, aR, 0, 0, 0);
+          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.x, -hb.y, aI, 0, 0, 0);
+          aI = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.y, -hb.x, aI, 0, 0, 0);
+        }
+        const cpx<T> damp = L.damp[0];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * ti + 4 * gk + r, j = 16 * tj + gi;
+          acc[r] = mul_mi(mk<T>(aR[r], aI[r]));
+          if (i < d && j < d) cmsub(acc[r], damp, X[i * rs + j]);
+        }
+      }
+      // ---- wave 0: list of a + 1 into the other copy (nobody reads that copy during this ADO)
+      if (wv == 0 && has_next) wide_build_list(Ln, en_n, nent_n, lane, d, sorted, sC, sS, sSite);
+      // ---- rows into the row buffer (a further pass for lists longer than the registers hold)
+#pragma unroll
+      for (int q = 0; q < PQA_WIDE_ROWS; ++q) {
+        const int idx = tid + kThreads * q;
+        if (idx < nval) R[idx + (int)(((uint32_t)idx * dmag) >> 20)] = v[q];
+      }
+      for (int idx = tid + kThreads * PQA_WIDE_ROWS; idx < nval; idx += kThreads) {
+        const int t = (int)(((uint32_t)idx * dmag) >> 20), jj = idx - t * d;
+        R[idx + t] = ldc<T>(xi, (int64_t)L.nb[t] * d2 + L.ste[t] * d + jj);
+      }
+      // y of this thread's upper elements from the FP32 stage input (stages 0-2), before the tile
+      // takes the own matrix's place
+      cpx<T> yo[3];
+      if (STAGE <= 2) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const int pk = tid + kThreads * q;
+          if (pk < np) {
+            const int ij = sIJ[pk];
+            yo[q] = X[(ij >> 8) * rs + (ij & 255)];
+          }
+        }
+      }
+      __syncthreads();  // every read of the own matrix is done; the row buffer is complete
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * ti + 4 * gk + r, j = 16 * tj + gi;
+        if (i < d && j < d) X[i * rs + j] = acc[r];
+      }
+      __syncthreads();
+      // ---- neighbour terms of the upper elements: row term of site i, column term of site j read as
+      // conjugated rows (the diagonal keeps the row value itself)
+      cpx<T> au[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int pk = tid + kThreads * q;
+        if (pk < np) {
+          const int ij = sIJ[pk], i = ij >> 8, j = ij & 255;
+          cpx<T> o = X[i * rs + j];
+          for (int t = L.gs[i]; t < L.gs[i + 1]; ++t) cmac(o, L.cl[t], R[t * rs + j]);
+          for (int t = L.gs[j]; t < L.gs[j + 1]; ++t) cmac(o, L.cr[t], i != j ? conjg(R[t * rs + i]) : R[t * rs + i]);
+          au[q] = o;
+        }
+      }
+      __syncthreads();  // every tile read is done: the tile now takes the stage output
+      const bool save_ado = STAGE == 3 && a == 0 && p.traj && p.save_slot >= 0;
+      const int64_t tbase = p.save_slot * p.traj_stride + I.traj_off;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int pk = tid + kThreads * q;
+        if (pk < np) {
+          const int ij = sIJ[pk], i = ij >> 8, j = ij & 255;
+          PreM w = pm[q];
+          if (STAGE <= 2) w.y = y_from_input_m<STAGE>(yo[q], w);
+          const double2 o = rk4_upper_m<STAGE>(p, I.poff + (int64_t)a * I.dp + pk, (double)au[q].x, (double)au[q].y, w);
+          X[i * rs + j] = mk<T>((T)o.x, (T)o.y);
+          if (i != j) X[j * rs + i] = mk<T>((T)o.x, (T)(-o.y));
+          if (save_ado) {
+            p.traj[tbase + i * d + j] = o;
+            if (i != j) p.traj[tbase + j * d + i] = make_double2(o.x, -o.y);
+          }
+        }
+      }
+      __syncthreads();
+      // ---- the full-matrix stage output (the next stage's input; after stage 3 the next step's)
+      for (int e = tid; e < d2; e += kThreads) {
+        const cpx<T> u = X[e + (int)(((uint32_t)e * dmag) >> 20)];
+        st(xout + I.off + base + e, mk<T>(u.x, u.y));
+      }
     }
   }
 }
@@ -1745,7 +2027,8 @@ struct pyqed_heom_plan {
   std::vector<size_t> res_extra;  // per instance: resident_extra_bytes
   int64_t tile_target = 0;
   int blocks_per_cu = 8;
-  int variant = 0;  // 0: site-projector kernel; 1: pipelined kernel for the MFMA class (Hermitian, FP32)
+  int variant = 0;  // bit 0: pipelined kernel for the MFMA class (Hermitian full-matrix FP32); bit 1: block
+                    // kernel for the wide class (packed FP32)
   int rowbuf = 1, colbuf = 0;  // projector path: max entries*d over instances (columns: non-Hermitian)
   // LDS dimensions of the projector kernels per item class (1: VALU, 2: MFMA commutator), so that
   // a class of small instances is not sized by the largest
@@ -2297,7 +2580,7 @@ extern "C" int pyqed_heom_plan_set_tiling(pyqed_heom_plan* P, int64_t target_ele
 }
 
 extern "C" int pyqed_heom_plan_set_variant(pyqed_heom_plan* P, int32_t variant) {
-  if (!P || variant < 0 || variant > 1) return fail(PYQED_EINVAL, "plan_set_variant: variant must be 0 or 1");
+  if (!P || variant < 0 || variant > 3) return fail(PYQED_EINVAL, "plan_set_variant: variant must be in 0..3");
   P->variant = variant;
   return PYQED_OK;
 }
@@ -2480,13 +2763,18 @@ int launch_stage_ti(pyqed_heom_plan* P, const StageParams& p0, hipStream_t st, i
     if (c == 0 || plds > (size_t)P->max_lds) {
       rc = launch(heom_stage_kernel<T, TI, T, S, PACK>, grid, stream_lds_bytes(P, sizeof(T)), st, p);
     } else if constexpr (PACK) {  // packed plans are Hermitian
+      const size_t wlds = c == 3 ? proj_block_bytes(P->pd[3].d2pad, P->pd[3].Kmax, P->pd[3].Lmax, sizeof(float), true) +
+                                       wide_ado_bytes(P->pd[3].d2pad, P->pd[3].rowbuf)
+                                 : 0;
       if (c == 2 && F32)
         rc = launch(heom_proj_kernel<T, TI, T, S, F32, true, true, 1>, grid, plds, st, p);
+      else if (c == 3 && F32 && (P->variant & 2) && wlds <= (size_t)P->max_lds)
+        rc = launch(heom_wide_kernel<S>, grid, wlds, st, p);
       else if (c == 3 && F32)
         rc = launch(heom_proj_kernel<T, TI, T, S, F32, true, true, F32 ? 2 : 1>, grid, plds, st, p);
       else
         rc = launch(heom_proj_kernel<T, TI, T, S, false, true, true, 1>, grid, plds, st, p);
-    } else if (P->hermitian && c == 2 && F32 && P->variant == 1 && pipe_lds_bytes(P, c) <= (size_t)P->max_lds) {
+    } else if (P->hermitian && c == 2 && F32 && (P->variant & 1) && pipe_lds_bytes(P, c) <= (size_t)P->max_lds) {
       rc = launch(heom_pipe_kernel<S>, grid, pipe_lds_bytes(P, c), st, p);
     } else if (P->hermitian) {
       if (c == 2 && F32)

@@ -19,7 +19,8 @@ from .. import _lib
 TUNING_FILE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tuning.json")
 TARGETS = (2048, 4096, 8192, 16384)
 BLOCKS_PER_CU = (4, 8, 12, 16)
-VARIANTS = (0, 1)  # 1, the pipelined MFMA-class kernel, exists for Hermitian full-matrix plans only
+VARIANTS = (0, 1, 2)  # bit 0: pipelined MFMA-class kernel (Hermitian full-matrix plans); bit 1: block kernel
+#                      for the wide class (packed plans with 17 <= d <= 32)
 
 
 def signature_of(arch, cus, precision, hermitian, dense, shapes, packed):
@@ -62,8 +63,9 @@ def tune(engine, dt, precision="mixed", targets=TARGETS, blocks_per_cu=BLOCKS_PE
     with milliseconds per step."""
     torch = engine.torch
     timings = {}
-    if engine.packed or not engine.hermitian:
-        variants = tuple(v for v in variants if v == 0) or (0,)
+    wide = engine.classes()["items"][3] > 0
+    keep = {0} | ({1} if engine.hermitian and not engine.packed else set()) | ({2} if engine.packed and wide else set())
+    variants = tuple(v for v in variants if v in keep) or (0,)
     for v in variants:
         for t in targets:
             for b in blocks_per_cu:

@@ -112,3 +112,34 @@ def test_launch_timing_counts_every_stage_launch():
             assert (ms[nc * s + c] > 0) == bool(items[c])
     eng.advance(1, 0.01, "mixed")
     assert sum(eng.read_timing()[1]) == 0  # recording stopped
+
+
+@pytest.mark.parametrize("d,L", [(17, 2), (24, 2), (31, 1)])
+def test_wide_block_variant(d, L):
+    """The block-cooperative wide kernel (variant bit 1: one ADO per block, one MFMA tile per wavefront)
+    against the oracle from a random Hermitian state, and against the default wavefront form over 20
+    steps (packed mixed runs)."""
+    from pyqed_amd.heom.engine import HeomEngine
+    inst = _inst(d, L, seed=d + 11)
+    rng = np.random.default_rng(d)
+    probe = HeomEngine([inst], resident=False)
+    assert probe.packed and probe.classes()["items"][3] > 0
+    nmax = int(probe.nmax[0])
+    A = (rng.normal(size=(nmax, d, d)) + 1j * rng.normal(size=(nmax, d, d))) * 0.1
+    state = A + np.conj(np.transpose(A, (0, 2, 1)))
+    ref = _oracle_step(inst, state, 0.05)
+    outs = {}
+    for variant in (0, 2):
+        eng = HeomEngine([inst], resident=False)
+        eng.autotune_cache = False
+        eng.set_tiling(2048, 8, variant)
+        eng.set_state([state])
+        eng.advance(1, 0.05, "mixed")
+        err = np.abs(eng.hierarchy(0) - ref).max() / np.abs(ref).max()
+        print(f"d={d} variant {variant}: one step vs oracle {err:.2e}")
+        assert err <= 1e-5
+        e2 = HeomEngine([inst], resident=False)
+        e2.autotune_cache = False
+        e2.set_tiling(2048, 8, variant)
+        outs[variant] = e2.run(0.02, 20, precision="mixed")[0]
+    assert np.abs(outs[0] - outs[2]).max() <= 1e-6
