@@ -16,10 +16,12 @@ Printed JSON line:
                  64 instances covering every (d, depth) stratum; after_timed_steps: the same over
                  every instance after the timed steps
   roofline       the dominant stage kernel (the kernel-class launch with the longest average time,
-                 timed per launch with HIP events on its stream): algorithmic HBM bytes per launch /
-                 average launch time, vs 8 TB/s; `step` gives the same for all stage kernels of a
-                 step; `traffic` the measured HBM bytes per launch from the committed counter summary
-                 when it was taken at this workload, state layout and tiling
+                 timed per launch with HIP events on its stream, in a pass of up to 10 steps after
+                 the timed steps): algorithmic HBM bytes per launch /
+                 average launch time, vs 8 TB/s; `launches` gives time, bytes and fraction for every
+                 (stage, class) launch, `step` the same for all stage kernels of a step; `traffic`
+                 the measured HBM bytes per launch from the committed counter summary when it was
+                 taken at this workload, state layout and tiling
   cpu_baseline   numpy FP64 restatement (the oracle) on the host's cores (R1: all cores, vectorised;
                  R0: one core, the reference's per-ADO loop) on a stratified sample, with the CPU model
 
@@ -987,9 +989,6 @@ def bench_heom(args, torch):
         eng.advance_graph(args.graph, dt, args.precision, chunk=args.graph)  # capture (+ one replay) untimed
         torch.cuda.synchronize()
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    timing = not args.graph and not args.no_launch_timing
-    if timing:
-        eng.set_timing(True)  # event pair around every kernel-class launch, on the launch stream
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     evs[0].record()
@@ -1009,6 +1008,15 @@ def bench_heom(args, torch):
     tiling_cached = bool(getattr(eng, "_tuned_hit", False))
     value = eng.n_ados * args.steps / wall
     print(f"[bench] timed {args.steps} steps in {wall:.3f}s", file=sys.stderr, flush=True)
+    mixed_final = eng.reduced_states()  # the state after warmup + timed steps (accuracy leg)
+    # per-launch times: an event pair around every kernel-class launch, on the launch stream, in a
+    # separate pass after the timed steps (events would lengthen a launch-bound step)
+    timing = not args.graph and not args.no_launch_timing
+    n_instr = min(args.steps, 10)
+    if timing:
+        eng.set_timing(True)
+        eng.advance(n_instr, dt, args.precision)
+        torch.cuda.synchronize()
     ts = 8 if args.precision == "mixed" else 16
     F = eng.classes()["elems"]
     S = eng.class_state()
@@ -1036,13 +1044,17 @@ def bench_heom(args, torch):
                          "bytes_per_launch": b, "achieved": b / (avg[k] / 1e3) / 1e9,
                          "frac": b / (avg[k] / 1e3) / 1e9 / HBM_PEAK_GBS,
                          "layout_bytes_per_launch": bl, "layout_achieved": bl / (avg[k] / 1e3) / 1e9,
-                         "launch_ms_all": {f"s{kk // nc}c{kk % nc}": v for kk, v in sorted(avg.items())}})
+                         "launches": {f"s{kk // nc}c{kk % nc}": {
+                             "ms": v, "bytes": alg_bytes(kk // nc, ts, F[kk % nc]),
+                             "frac": alg_bytes(kk // nc, ts, F[kk % nc]) / (v / 1e3) / 1e9 / HBM_PEAK_GBS}
+                             for kk, v in sorted(avg.items())}})
         roofline["traffic"] = measured_traffic(name, args.workload, desc, eng.packed, tiling)
     else:
         roofline.update({"kernel": "all stage kernels of a step" if eng.stats()["stream_items"] else
                          "heom_resident_kernel (every instance LDS-resident)",
                          "achieved": roofline["step"]["achieved"], "frac": roofline["step"]["frac"]})
-    mixed_final = eng.reduced_states()
+    if avg:
+        roofline["instrumented_steps"] = n_instr
     packed = eng.packed
     eng.close()
     del eng

@@ -84,7 +84,12 @@ struct WorkItem {
   int32_t inst, ado0, nado, pad;
 };
 
-enum { F_SDIP = 1, F_CDIP = 2, F_DENSE = 4, F_HERM = 8, F_SITESORT = 16 };  // F_SITESORT: site(k) nondecreasing
+// F_SITESORT: site(k) nondecreasing; F_TINY: at most kThreads elements, d <= kTinyD, diagonal couplings and
+// at most kTinyEnt neighbour entries per ADO (the resident kernel keeps one element per thread in registers);
+// F_TINY4: at most 4 neighbour entries per ADO
+enum { F_SDIP = 1, F_CDIP = 2, F_DENSE = 4, F_HERM = 8, F_SITESORT = 16, F_TINY = 32, F_TINY4 = 64 };
+constexpr int kTinyD = 8;
+constexpr int kTinyEnt = 8;
 
 // Plan tables (instances, work items, row pointers) are never written while a kernel runs. Read
 // through the constant address space at a wave-uniform index they become scalar loads, which
@@ -1799,66 +1804,190 @@ __global__ __launch_bounds__(kThreads) void heom_resident_kernel(StageParams p) 
   double fnext[6] = {0, 0, 0, 0, 0, 0};
   if (driven && p.nsteps > 0)
     for (int c = 0; c < 6; ++c) fnext[c] = p.field[6 * p.step + c];
-  for (int64_t s = 0; s < p.nsteps; ++s) {
-    double f[6];
+  auto next_field = [&](int64_t s, double* f) {
     for (int c = 0; c < 6; ++c) f[c] = fnext[c];
     if (driven && s + 1 < p.nsteps)
       for (int c = 0; c < 6; ++c) fnext[c] = p.field[6 * (p.step + s + 1) + c];
-#pragma unroll 1
-    for (int stage = 0; stage < 4; ++stage) {
-      const int col = stage_col(stage);
-      if (driven && stage != 2) {  // stages 1 and 2 share the time t + dt/2
-        __syncthreads();            // every read of the previous operators is done
-        set_ops(f[col], f[3 + col]);
-      }
-      __syncthreads();  // operators and the previous stage's output are in place
-      const double fc = f[3 + col];
-      const cpx<T>* xs = stage == 1 ? X0 : (stage == 2 ? X1 : X0);
-      cpx<T>* xd = stage == 1 ? X1 : X0;
-      for (int e = tid; e < n; e += nth) {
-        const int a = e / d2, r = e - a * d2;
-        const int i = r / d, j = r - (r / d) * d;
-        // Hermitian (packed) plans: every ADO stays Hermitian, so only upper elements are computed
-        // and the lower ones are written as their conjugates
-        if (PACK && i > j) continue;
-        const int eT = a * d2 + j * d + i;
-        cpx<T> rr;
-        if (stage == 0) {
-          // RHS input is y (double), converted to T as it is read
-          rr = rhs(Y + a * d2, Y, a, i, j, fc);
-        } else {
-          rr = rhs(xs + a * d2, xs, a, i, j, fc);
-        }
-        const double rx = (double)rr.x, ry = (double)rr.y;
-        const cpx<double> yv = Y[e];
-        if (stage == 0) {
-          Yn[e] = mk(yv.x + rx * (dt / 6), yv.y + ry * (dt / 6));
-          const cpx<T> o = mk<T>((T)(yv.x + rx * (dt / 2)), (T)(yv.y + ry * (dt / 2)));
-          xd[e] = o;
-          if (PACK && eT != e) xd[eT] = conjg(o);
-        } else if (stage < 3) {
-          const cpx<double> an = Yn[e];
-          Yn[e] = mk(an.x + rx * (dt / 3), an.y + ry * (dt / 3));
-          const double h = stage == 1 ? dt / 2 : dt;
-          const cpx<T> o = mk<T>((T)(yv.x + rx * h), (T)(yv.y + ry * h));
-          xd[e] = o;
-          if (PACK && eT != e) xd[eT] = conjg(o);
-        }
-        if (stage == 3) {
-          const cpx<double> an = Yn[e];
-          // Y is not read by other threads in stage 3 (its input is X0), so it can be updated in place
-          const cpx<double> yn = mk(an.x + rx * (dt / 6), an.y + ry * (dt / 6));
-          Y[e] = yn;
-          if (PACK && eT != e) Y[eT] = conjg(yn);
-        }
-      }
-    }
-    __syncthreads();
-    if (p.traj && ((s + 1) % p.save_every) == 0) {
+  };
+  int64_t since_save = 0;
+  auto save_sample = [&](int64_t s) {  // after step s, once every save_every steps
+    if (p.traj && ++since_save == p.save_every) {
+      since_save = 0;
       const int64_t slot = (s + 1) / p.save_every - 1;
-      for (int e = tid; e < d2; e += nth) {
-        p.traj[slot * p.traj_stride + I.traj_off + e] = make_double2(Y[e].x, Y[e].y);
+      for (int e = tid; e < d2; e += nth) p.traj[slot * p.traj_stride + I.traj_off + e] = make_double2(Y[e].x, Y[e].y);
+    }
+  };
+  if (I.flags & F_TINY) {
+    // Tiny instances: thread e owns element e for the whole run.  Its indices, damping, the element
+    // offsets of its neighbour entries and their coefficients sit in registers; d and the entry count
+    // are bounded at compile time (D, E) and the four stages are unrolled, so a stage is one round of
+    // independent LDS loads (row i and column j of the input, the neighbours' (i, j) elements) and
+    // its arithmetic.  Coefficients are formed once Q is in LDS, and again per stage only when a
+    // coupling dipole makes Q time dependent.
+    const int te = tid < n ? tid : 0;
+    const int ta = te / d2, tr = te - ta * d2, ti = tr / d, tj = tr - ti * d;
+    const bool tact = tid < n && !(PACK && ti > tj);
+    const int teT = ta * d2 + tj * d + ti;
+    const int tt0 = sRow[ta], tcnt = tact ? sRow[ta + 1] - tt0 : 0;
+    const cpx<T> tdamp = sDamp[ta];
+    const bool qvar = driven && cd;
+    auto tiny_run = [&](auto DC, auto EC) {
+      constexpr int D = decltype(DC)::value, E = decltype(EC)::value;
+      int nb[E];
+      cpx<T> cf[E];
+#pragma unroll
+      for (int u = 0; u < E; ++u) {
+        nb[u] = te;
+        cf[u] = mk<T>(0, 0);
       }
+      auto coefs = [&]() {
+#pragma unroll
+        for (int u = 0; u < E; ++u) {
+          const int t = u < tcnt ? tt0 + u : 0;
+          const int2 rc = sRec[t];
+          const cpx<T> c = sCL[t] * l.Q[rc.y + ti] + sCR[t] * l.Q[rc.y + tj];
+          cf[u] = u < tcnt ? c : mk<T>(0, 0);
+          nb[u] = u < tcnt ? rc.x * d2 + ti * d + tj : te;
+        }
+      };
+      if (nent > 0 && !driven) coefs();
+      // right-hand side of this thread's element from input xs (every index in bounds; unused terms
+      // are dropped by selects, not branches, so the loads issue together)
+      auto rhs_t = [&](const auto* xs) {
+        const auto* own = xs + ta * d2;
+        cpx<T> c = mk<T>(0, 0);
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          const int kk = k < d ? k : 0;
+          const cpx<T> h1 = l.H[ti * d + kk], x1 = ldc<T>(own, kk * d + tj);
+          const cpx<T> x2 = ldc<T>(own, ti * d + kk), h2 = l.H[kk * d + tj];
+          if (k < d) {
+            cmac(c, h1, x1);
+            cmsub(c, x2, h2);
+          }
+        }
+        cpx<T> r = mul_mi(c);
+        cmsub(r, tdamp, ldc<T>(own, tr));
+#pragma unroll
+        for (int u = 0; u < E; ++u) {
+          const cpx<T> v = ldc<T>(xs, nb[u]);
+          if (u < tcnt) cmac(r, cf[u], v);
+        }
+        return r;
+      };
+      auto stage_t = [&](auto SC, const double* f, bool first) {
+        constexpr int STG = decltype(SC)::value;
+        constexpr int col = STG == 0 ? 0 : (STG == 3 ? 2 : 1);
+        if (driven && STG != 2) {  // stages 1 and 2 share the time t + dt/2
+          __syncthreads();          // every read of the previous operators is done
+          set_ops(f[col], f[3 + col]);
+        }
+        __syncthreads();  // operators and the previous stage's output are in place
+        if (driven && nent > 0 && (qvar ? STG != 2 : (first && STG == 0))) coefs();
+        if (!tact) return;
+        cpx<T> rr;
+        if constexpr (STG == 0)
+          rr = rhs_t(Y);  // RHS input is y (double), converted to T as it is read
+        else
+          rr = rhs_t(STG == 2 ? X1 : X0);
+        const double rx = (double)rr.x, ry = (double)rr.y;
+        const cpx<double> yv = Y[te];
+        if constexpr (STG < 3) {
+          const cpx<double> an = STG == 0 ? yv : Yn[te];
+          constexpr double wa = STG == 0 ? 1.0 / 6 : 1.0 / 3;
+          Yn[te] = mk(an.x + rx * (dt * wa), an.y + ry * (dt * wa));
+          const double h = STG == 2 ? dt : dt / 2;
+          const cpx<T> o = mk<T>((T)(yv.x + rx * h), (T)(yv.y + ry * h));
+          cpx<T>* xd = STG == 1 ? X1 : X0;
+          xd[te] = o;
+          if (PACK && teT != te) xd[teT] = conjg(o);
+        } else {
+          // Y is not read by other threads in stage 3 (its input is X0), so it is updated in place
+          const cpx<double> an = Yn[te];
+          const cpx<double> yn = mk(an.x + rx * (dt / 6), an.y + ry * (dt / 6));
+          Y[te] = yn;
+          if (PACK && teT != te) Y[teT] = conjg(yn);
+        }
+      };
+      for (int64_t s = 0; s < p.nsteps; ++s) {
+        double f[6];
+        next_field(s, f);
+        stage_t(std::integral_constant<int, 0>(), f, s == 0);
+        stage_t(std::integral_constant<int, 1>(), f, s == 0);
+        stage_t(std::integral_constant<int, 2>(), f, s == 0);
+        stage_t(std::integral_constant<int, 3>(), f, s == 0);
+        __syncthreads();
+        save_sample(s);
+      }
+    };
+    using I4 = std::integral_constant<int, 4>;
+    using I8 = std::integral_constant<int, 8>;
+    const bool e4 = (I.flags & F_TINY4) != 0;
+    if (d <= 2) {
+      if (e4) tiny_run(std::integral_constant<int, 2>(), I4());
+      else tiny_run(std::integral_constant<int, 2>(), I8());
+    } else if (d <= 4) {
+      if (e4) tiny_run(I4(), I4());
+      else tiny_run(I4(), I8());
+    } else {
+      if (e4) tiny_run(I8(), I4());
+      else tiny_run(I8(), I8());
+    }
+  } else {
+    for (int64_t s = 0; s < p.nsteps; ++s) {
+      double f[6];
+      next_field(s, f);
+#pragma unroll 1
+      for (int stage = 0; stage < 4; ++stage) {
+        const int col = stage_col(stage);
+        if (driven && stage != 2) {  // stages 1 and 2 share the time t + dt/2
+          __syncthreads();            // every read of the previous operators is done
+          set_ops(f[col], f[3 + col]);
+        }
+        __syncthreads();  // operators and the previous stage's output are in place
+        const double fc = f[3 + col];
+        const cpx<T>* xs = stage == 1 ? X0 : (stage == 2 ? X1 : X0);
+        cpx<T>* xd = stage == 1 ? X1 : X0;
+        for (int e = tid; e < n; e += nth) {
+          const int a = e / d2, r = e - a * d2;
+          const int i = r / d, j = r - (r / d) * d;
+          // Hermitian (packed) plans: every ADO stays Hermitian, so only upper elements are computed
+          // and the lower ones are written as their conjugates
+          if (PACK && i > j) continue;
+          const int eT = a * d2 + j * d + i;
+          cpx<T> rr;
+          if (stage == 0) {
+            // RHS input is y (double), converted to T as it is read
+            rr = rhs(Y + a * d2, Y, a, i, j, fc);
+          } else {
+            rr = rhs(xs + a * d2, xs, a, i, j, fc);
+          }
+          const double rx = (double)rr.x, ry = (double)rr.y;
+          const cpx<double> yv = Y[e];
+          if (stage == 0) {
+            Yn[e] = mk(yv.x + rx * (dt / 6), yv.y + ry * (dt / 6));
+            const cpx<T> o = mk<T>((T)(yv.x + rx * (dt / 2)), (T)(yv.y + ry * (dt / 2)));
+            xd[e] = o;
+            if (PACK && eT != e) xd[eT] = conjg(o);
+          } else if (stage < 3) {
+            const cpx<double> an = Yn[e];
+            Yn[e] = mk(an.x + rx * (dt / 3), an.y + ry * (dt / 3));
+            const double h = stage == 1 ? dt / 2 : dt;
+            const cpx<T> o = mk<T>((T)(yv.x + rx * h), (T)(yv.y + ry * h));
+            xd[e] = o;
+            if (PACK && eT != e) xd[eT] = conjg(o);
+          }
+          if (stage == 3) {
+            const cpx<double> an = Yn[e];
+            // Y is not read by other threads in stage 3 (its input is X0), so it can be updated in place
+            const cpx<double> yn = mk(an.x + rx * (dt / 6), an.y + ry * (dt / 6));
+            Y[e] = yn;
+            if (PACK && eT != e) Y[eT] = conjg(yn);
+          }
+        }
+      }
+      __syncthreads();
+      save_sample(s);
     }
   }
   __syncthreads();
@@ -2435,6 +2564,14 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     if (mat.size() > (size_t)INT32_MAX || coef.size() > (size_t)INT32_MAX || coefp.size() > (size_t)INT32_MAX) {
       destroy_plan(P);
       return fail(PYQED_EINVAL, "coefficient pools exceed 2^31 entries");
+    }
+    {
+      int maxrow = 0;
+      for (int a = 0; a < t.nmax; ++a) maxrow = std::max(maxrow, t.rowptr[a + 1] - t.rowptr[a]);
+      if (!dense && s.d <= kTinyD && (int64_t)t.nmax * I.d2 <= kThreads && maxrow <= kTinyEnt) {
+        I.flags |= F_TINY;
+        if (maxrow <= 4) I.flags |= F_TINY4;
+      }
     }
     P->inst.push_back(I);
     P->nmax.push_back(t.nmax);

@@ -13,6 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 @pytest.mark.skipif(not (shutil.which("hipcc") or os.path.exists("/opt/rocm/bin/hipcc")), reason="needs hipcc")
+@pytest.mark.skipif(not os.path.exists(os.path.join(HERE, "sanitize", "build.sh")),
+                    reason="sanitizer build files not present (they are not shipped to GPU runs)")
 def test_host_code_under_address_and_ub_sanitizers(tmp_path):
     build = subprocess.run(["bash", os.path.join(HERE, "sanitize", "build.sh"), str(tmp_path)],
                            capture_output=True, text=True, timeout=900)
