@@ -793,11 +793,6 @@ void heom_proj_kernel(StageParams p) {
     // LDS row stride of the own matrix, H and the neighbour rows: d+1 (odd), so that column reads of
     // the A operand (stride rs) and the rows of the row buffer fall on different banks
     const int rs = d + 1;
-    __syncthreads();
-    proj_header<T>(p, I, col, dmag, sH, sC, sSite, sS, PACK ? sIJ : nullptr);
-    __syncthreads();
-    // MFMA lane coordinates: lane = (gk, gi), gk = lane / 16, gi = lane % 16
-    const int gi = lane & 15, gk = lane >> 4;
     const cpx_store<TI>* xi = xin + I.off;
     // Software pipeline over this wave's ADOs: the neighbour list (and, on the MFMA path, the own
     // matrix) of the next ADO are loaded while the current one is processed; they are issued after
@@ -818,7 +813,14 @@ void heom_proj_kernel(StageParams p) {
         if (e < d2) xt_n[q] = xi[(int64_t)an * d2 + e];
       }
     };
+    // the first ADO's list and matrix are requested before the instance header, so the two round
+    // trips overlap (the prefetch touches registers only)
     if (wi.ado0 + wv < aend) prefetch(wi.ado0 + wv);
+    __syncthreads();
+    proj_header<T>(p, I, col, dmag, sH, sC, sSite, sS, PACK ? sIJ : nullptr);
+    __syncthreads();
+    // MFMA lane coordinates: lane = (gk, gi), gk = lane / 16, gi = lane % 16
+    const int gi = lane & 15, gk = lane >> 4;
     for (int a = wi.ado0 + wv; a < aend; a += kThreads / 64) {
       const int64_t base = (int64_t)a * d2;
       // ---- neighbour list and own matrix (prefetched)
