@@ -2291,7 +2291,7 @@ std::complex<double> cz(const double* p, int i) { return std::complex<double>(p[
 // persistent grid of the streamed kernels.
 void retile(pyqed_heom_plan* P, int64_t target, int blocks_per_cu) {
   if (target <= 0)
-    target = std::min<int64_t>(8192, std::max<int64_t>(256, P->stream_elems / std::max<int64_t>(1, (int64_t)P->num_cu * 8)));
+    target = std::min<int64_t>(8192, std::max<int64_t>(64, P->stream_elems / std::max<int64_t>(1, (int64_t)P->num_cu * 8)));
   P->tile_target = target;
   P->blocks_per_cu = std::max(1, std::min(blocks_per_cu, 32));
   P->items.clear();
