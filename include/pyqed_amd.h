@@ -157,6 +157,14 @@ int pyqed_heom_run(pyqed_heom_plan* plan, int32_t precision, void* y, void* work
  * generator. */
 int pyqed_heom_rhs(pyqed_heom_plan* plan, const void* x, void* out, const double* field, int64_t step,
                    int32_t col, void* stream);
+/* The right-hand side r = RHS(x) of pyqed_heom_rhs (same layout, step and column), combined in the same
+ * launch (no counterpart in the reference; for the stage combinations of the RK4 adjoint and other
+ * Krylov-type callers): out = base + alpha r when base is given, else out = r (out may be NULL when
+ * acc is given); acc = acc + beta r, or beta r when acc_init is nonzero (acc may be NULL).  out and
+ * acc must not alias x, nor out the base. */
+int pyqed_heom_rhs_combine(pyqed_heom_plan* plan, const void* x, void* out, const void* base, double alpha, void* acc,
+                           double beta, int32_t acc_init, const double* field, int64_t step, int32_t col,
+                           void* stream);
 
 /* Kernel-level statistics of the last run (for the benchmark): number of
  * streamed work items, resident instances, streamed elements, resident

@@ -140,6 +140,12 @@ struct StageParams {
   int32_t rowbuf, colbuf;  // projector path: per-wave neighbour row / column buffer (complex elements)
   int32_t d2max, qmax, Kmax, Lmax;
   int32_t d2pad;  // projector path: max d*(d+1), matrices in LDS with row stride d+1
+  // right-hand-side evaluation (STAGE 4) with combinations, r = L x: xout = cbase + calpha r when
+  // cbase is set (else xout = r, when xout is set); cacc = (cinit ? 0 : cacc) + cbeta r when cacc is set
+  const double2* cbase;
+  double2* cacc;
+  double calpha, cbeta;
+  int32_t cinit;
 };
 
 // LDS carve-up shared by both kernels.
@@ -445,7 +451,16 @@ __device__ __forceinline__ void rk4_store(const StageParams& p, int64_t gx, int6
                                           int64_t tposT) {
   const double dt = p.dt;
   if (STAGE == 4) {
-    ((double2*)p.xout)[gx] = make_double2(rx, ry);
+    if (p.cacc) {
+      const double2 a = p.cinit ? make_double2(0, 0) : p.cacc[gx];
+      p.cacc[gx] = make_double2(a.x + p.cbeta * rx, a.y + p.cbeta * ry);
+    }
+    if (p.cbase) {
+      const double2 b = p.cbase[gx];
+      ((double2*)p.xout)[gx] = make_double2(b.x + p.calpha * rx, b.y + p.calpha * ry);
+    } else if (p.xout) {
+      ((double2*)p.xout)[gx] = make_double2(rx, ry);
+    }
   } else if (STAGE == 0) {
     if (!s.lower) {
       if (DELTA) st_reg<DELTA>(p, s.g, rx * (dt / 6), ry * (dt / 6));
@@ -2512,7 +2527,7 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     }
     // ---- site-projector path: every coupling is c_m |p_m><p_m| (diagonal with one nonzero entry)
     I.path = 0;
-    if (!dense && !s.cdiag && s.d >= 6 && !(flags & PYQED_PLAN_GENERIC) && !P->adjoint) {
+    if (!dense && !s.cdiag && s.d >= 6 && !(flags & PYQED_PLAN_GENERIC)) {
       std::vector<int> psite(s.ncoup, -1);
       std::vector<std::complex<double>> pw(s.ncoup);
       bool proj = true;
@@ -2523,7 +2538,7 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
           if (q != 0.0) {
             ++nz;
             psite[m] = e;
-            pw[m] = q;
+            pw[m] = P->adjoint ? std::conj(q) : q;  // the adjoint plan's couplings are Q^H
           }
         }
         proj = nz == 1;
@@ -2853,7 +2868,7 @@ struct LaunchTimer {
   hipStream_t st;
   int k = -1;
   LaunchTimer(pyqed_heom_plan* P_, hipStream_t st_, int tag) : P(P_), st(st_) {
-    if (!P->timing) return;
+    if (!P->timing || tag < 0) return;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
     k = (int)P->ev_tag.size();
@@ -2897,7 +2912,7 @@ int launch_stage_ti(pyqed_heom_plan* P, const StageParams& p0, hipStream_t st, i
     // the projector kernel keeps neighbour rows (and columns) in LDS; when they do not fit, the
     // generic kernel (which any instance can use) takes the class
     const size_t plds = c == 0 ? 0 : proj_lds_bytes(P, c, sizeof(T));
-    LaunchTimer timer(P, st, kClasses * S + c);
+    LaunchTimer timer(P, st, S < 4 ? kClasses * S + c : -1);  // right-hand-side launches are not timed
     constexpr bool F32 = std::is_same<T, float>::value;
     if (c == 0 || plds > (size_t)P->max_lds) {
       rc = launch(heom_stage_kernel<T, TI, T, S, PACK>, grid, stream_lds_bytes(P, sizeof(T)), st, p);
@@ -3247,6 +3262,31 @@ extern "C" int pyqed_heom_adjoint_contract(pyqed_heom_plan* P, const void* const
   return PYQED_OK;
 }
 
+namespace {
+int rhs_launch(pyqed_heom_plan* P, StageParams p, int32_t col, void* stream) {
+  p.item_begin = 0;
+  p.item_end = (int)P->items.size();
+  p.use_xcd = 0;
+  p.col_override = col;
+  const size_t lds = stream_lds_bytes(P, sizeof(double));
+  if (!P->hermitian && P->n_proj > 0) {
+    // streamed items through their class kernels (the projector classes on the FP64 projector
+    // kernel, rows and columns of the neighbours from the input), then the LDS-resident instances'
+    // items on the generic kernel.  Hermitian plans keep the generic kernel for every item: their
+    // projector kernels read columns as conjugated rows, which holds only for Hermitian inputs.
+    const int rc = launch_stage_ti<double, 4, double, false>(P, p, (hipStream_t)stream, 0, kClasses);
+    if (rc || P->cls_begin[kClasses] == (int)P->items.size()) return rc;
+    p.item_begin = P->cls_begin[kClasses];
+    p.use_xcd = 0;
+    const int nres = (int)P->items.size() - p.item_begin;
+    return launch(heom_stage_kernel<double, double, double, 4, false>, std::max(1, std::min(nres, P->num_cu * 8)), lds,
+                  (hipStream_t)stream, p);
+  }
+  const int grid = std::max(1, std::min((int)P->items.size(), P->num_cu * 8));
+  return launch(heom_stage_kernel<double, double, double, 4, false>, grid, lds, (hipStream_t)stream, p);
+}
+}  // namespace
+
 extern "C" int pyqed_heom_rhs(pyqed_heom_plan* P, const void* x, void* out, const double* field, int64_t step,
                               int32_t col, void* stream) {
   if (!P || !x || !out) return fail(PYQED_EINVAL, "heom_rhs: null argument");
@@ -3256,11 +3296,27 @@ extern "C" int pyqed_heom_rhs(pyqed_heom_plan* P, const void* x, void* out, cons
   p.step = step;
   p.xin = x;
   p.xout = out;
-  p.item_begin = 0;
-  p.item_end = (int)P->items.size();
-  p.use_xcd = 0;
-  p.col_override = col;
-  const int grid = std::max(1, std::min((int)P->items.size(), P->num_cu * 8));
-  const size_t lds = stream_lds_bytes(P, sizeof(double));
-  return launch(heom_stage_kernel<double, double, double, 4, false>, grid, lds, (hipStream_t)stream, p);
+  return rhs_launch(P, p, col, stream);
+}
+
+extern "C" int pyqed_heom_rhs_combine(pyqed_heom_plan* P, const void* x, void* out, const void* base, double alpha,
+                                      void* acc, double beta, int32_t acc_init, const double* field, int64_t step,
+                                      int32_t col, void* stream) {
+  if (!P || !x) return fail(PYQED_EINVAL, "heom_rhs_combine: null plan or input");
+  if (!out && !acc) return fail(PYQED_EINVAL, "heom_rhs_combine: neither out nor acc given");
+  if (base && !out) return fail(PYQED_EINVAL, "heom_rhs_combine: base given without out");
+  if (out == x || acc == x || (base && base == out))
+    return fail(PYQED_EINVAL, "heom_rhs_combine: out and acc must not alias the input, nor out the base");
+  if (col < 0 || col > 2) return fail(PYQED_EINVAL, "heom_rhs_combine: col must be 0, 1 or 2");
+  StageParams p = base_params(P);
+  p.field = field;
+  p.step = step;
+  p.xin = x;
+  p.xout = out;
+  p.cbase = (const double2*)base;
+  p.calpha = alpha;
+  p.cacc = (double2*)acc;
+  p.cbeta = beta;
+  p.cinit = acc_init ? 1 : 0;
+  return rhs_launch(P, p, col, stream);
 }

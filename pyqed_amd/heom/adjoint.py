@@ -67,6 +67,16 @@ class HeomAdjoint:
                                          None if field is None else field.data_ptr(), int(step), int(col),
                                          _lib.stream_ptr(self.torch)))
 
+    def _comb(self, eng, x, out, base, alpha, acc, beta, init, field, step, col):
+        """out = base + alpha L x (out, base optional) and acc = acc + beta L x (or beta L x when init),
+        in the right-hand-side launch."""
+        def ptr(t):
+            return None if t is None else t.data_ptr()
+        _lib.check(self.L.pyqed_heom_rhs_combine(eng.plan, x.data_ptr(), ptr(out), ptr(base), float(alpha), ptr(acc),
+                                                 float(beta), 1 if init else 0,
+                                                 None if field is None else field.data_ptr(), int(step), int(col),
+                                                 _lib.stream_ptr(self.torch)))
+
     def _to_work(self, k, m):
         """Caller-basis matrix -> working basis of instance k (the engine may propagate in a rotated basis)."""
         U = self.fwd.prep[k]["U"]
@@ -125,8 +135,8 @@ class HeomAdjoint:
             lam[o:o + d * d] = torch.as_tensor(self._to_work(k, bar).reshape(-1), device=dev)
         G = torch.zeros((max(nt, 1), 3, fwd.traj_stride), dtype=torch.complex128, device=dev)
         Ys = torch.zeros((min(ck, max(nt, 1)), n), dtype=torch.complex128, device=dev)
-        kf, x2, x3, x4 = z(), z(), z(), z()
-        K, X2, X3, X4, acc = z(), z(), z(), z(), z()
+        x2, x3, x4 = z(), z(), z()
+        X2, X3, X4, acc = z(), z(), z(), z()
         w = (C.c_double * 4)(dt / 6, dt / 3, dt / 3, dt / 6)
         xp = (C.c_void_p * 4)()
         up = (C.c_void_p * 4)()
@@ -143,26 +153,16 @@ class HeomAdjoint:
             for j in reversed(range(m)):
                 s = c0 + j
                 y = Ys[j]
-                # forward stage inputs of step s
-                self._rhs(fwd, y, kf, f, s, 0)
-                torch.add(y, kf, alpha=dt / 2, out=x2)
-                self._rhs(fwd, x2, kf, f, s, 1)
-                torch.add(y, kf, alpha=dt / 2, out=x3)
-                self._rhs(fwd, x3, kf, f, s, 1)
-                torch.add(y, kf, alpha=dt, out=x4)
-                # adjoint stages, times reversed
+                # forward stage inputs of step s: x2 = y + dt/2 k1, x3 = y + dt/2 k2, x4 = y + dt k3
+                self._comb(fwd, y, x2, y, dt / 2, None, 0.0, False, f, s, 0)
+                self._comb(fwd, x2, x3, y, dt / 2, None, 0.0, False, f, s, 1)
+                self._comb(fwd, x3, x4, y, dt, None, 0.0, False, f, s, 1)
+                # adjoint stages, times reversed: K1 = L^H u, X2 = u + dt/2 K1, ..., acc = K1 + 2 K2 + 2 K3 + K4
                 u = lam
-                self._rhs(adj, u, K, f, s, 2)
-                acc.copy_(K)
-                torch.add(u, K, alpha=dt / 2, out=X2)
-                self._rhs(adj, X2, K, f, s, 1)
-                acc.add_(K, alpha=2.0)
-                torch.add(u, K, alpha=dt / 2, out=X3)
-                self._rhs(adj, X3, K, f, s, 1)
-                acc.add_(K, alpha=2.0)
-                torch.add(u, K, alpha=dt, out=X4)
-                self._rhs(adj, X4, K, f, s, 0)
-                acc.add_(K)
+                self._comb(adj, u, X2, u, dt / 2, acc, 1.0, True, f, s, 2)
+                self._comb(adj, X2, X3, u, dt / 2, acc, 2.0, False, f, s, 1)
+                self._comb(adj, X3, X4, u, dt, acc, 2.0, False, f, s, 1)
+                self._comb(adj, X4, None, None, 0.0, acc, 1.0, False, f, s, 0)
                 # kbar_1 = dt/6 X4 pairs with x1 = y, kbar_2 = dt/3 X3 with x2, kbar_3 = dt/3 X2 with x3,
                 # kbar_4 = dt/6 u with x4
                 for i, (a, b) in enumerate(((y, X4), (x2, X3), (x3, X2), (x4, u))):

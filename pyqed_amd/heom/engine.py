@@ -407,9 +407,9 @@ class HeomEngine:
     def rhs(self, x, field=None, step=0, col=0):
         """FP64 right-hand side of full hierarchy matrices x (device complex128, every ADO of every
         instance as a full d x d matrix, in the layout of y without packing: instance starts rounded
-        up to multiples of 16 elements; also for packed engines)."""
+        up to multiples of 16 elements; also for packed engines).  Padding between instances is zero."""
         torch = self.torch
-        out = torch.empty_like(x)
+        out = torch.zeros_like(x)
         _lib.check(self.L.pyqed_heom_rhs(self.plan, x.data_ptr(), out.data_ptr(),
                                          None if field is None else field.data_ptr(), step, col,
                                          _lib.stream_ptr(torch)))

@@ -162,3 +162,110 @@ def test_adjoint_plan_is_the_transpose():
             assert abs(a - b) <= 1e-12 * max(abs(a), 1.0), (col, a, b)
         fw.close()
         ad.close()
+
+
+def _proj_instance(d, L, rng):
+    """A non-Hermitian site-projector instance: complex projector weights and bath exponents, a
+    driven dipole."""
+    H = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+    H = H + H.conj().T
+    sites = [0, d // 2, d - 1]
+    Q = np.array([np.diag(np.eye(d)[s]) * (0.7 + 0.4j * k) for k, s in enumerate(sites)]).astype(complex)
+    K = 2 * len(sites)
+    bath = (rng.normal(size=K) + 1j * rng.normal(size=K), rng.normal(size=K) + 1j * rng.normal(size=K),
+            rng.uniform(0.2, 1.0, K) + 0j, rng.uniform(0.5, 2.0, K) + 1j * rng.normal(size=K),
+            np.repeat(np.arange(len(sites)), 2))
+    sd = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+    return dict(H=H, Q=Q, bath=bath, L=L, rho0=np.eye(d) / d, sdip=sd)
+
+
+@pytest.mark.parametrize("d,L", [(7, 3), (13, 2), (20, 2)])
+def test_projector_rhs_forward_and_adjoint(d, L):
+    """Non-Hermitian plans evaluate the right-hand side on the projector kernels (VALU forms, FP64) for
+    site-projector couplings: forward and adjoint plans against the generic kernel at every stage
+    time, and <v, L u> = <L^H v, u> across the two."""
+    import torch
+    from pyqed_amd.heom.engine import HeomEngine
+    rng = np.random.default_rng(d)
+    inst = _proj_instance(d, L, rng)
+    eng = {(adj, gen): HeomEngine([inst], hermitian=False, packed=False, adjoint=adj, generic=gen, resident=False)
+           for adj in (False, True) for gen in (False, True)}
+    assert eng[False, False].classes()["items"][0] == 0 and eng[True, False].classes()["items"][0] == 0
+    assert eng[False, True].classes()["items"][0] > 0
+    dev = eng[False, False].device
+    n = eng[False, False].n_elems
+    u = torch.as_tensor(rng.normal(size=n) + 1j * rng.normal(size=n), device=dev)
+    v = torch.as_tensor(rng.normal(size=n) + 1j * rng.normal(size=n), device=dev)
+    field = torch.as_tensor(rng.normal(size=(2, 6)), device=dev)
+    m = int(eng[False, False].nmax[0]) * d * d
+    for col in (0, 1, 2):
+        out = {k: e.rhs(v if k[0] else u, field=field, step=1, col=col) for k, e in eng.items()}
+        for adj in (False, True):
+            a, b = out[adj, False][:m], out[adj, True][:m]
+            err = (a - b).abs().max().item() / b.abs().max().item()
+            print(f"d={d} col={col} adjoint={adj}: projector vs generic {err:.1e}")
+            assert err <= 1e-12
+        lhs = torch.vdot(v[:m], out[False, False][:m]).item()
+        rhs = torch.vdot(out[True, False][:m], u[:m]).item()
+        assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0), (col, lhs, rhs)
+    for e in eng.values():
+        e.close()
+
+
+def test_rhs_with_resident_and_projector_instances():
+    """A plan with an LDS-resident instance and a projector instance: the right-hand side covers both
+    (class kernels for the streamed items, the generic kernel for the resident ones)."""
+    import torch
+    from pyqed_amd.heom.engine import HeomEngine
+    rng = np.random.default_rng(5)
+    small = dict(H=np.array([[0.3, 0.2], [0.2, -0.3]], complex), Q=np.array([np.diag([1.0, -1.0])]).astype(complex),
+                 bath=(np.array([0.2 - 0.1j, 0.05]), np.array([0.2 + 0.1j, 0.05]), np.array([0.2, 0.05]) + 0j,
+                       np.array([1.0, 6.0]) + 0j, np.array([0, 0])), L=3, rho0=np.eye(2) / 2)
+    insts = [small, _proj_instance(7, 3, rng)]
+    a = HeomEngine(insts, hermitian=False, packed=False)
+    b = HeomEngine(insts, hermitian=False, packed=False, generic=True)
+    assert a.stats()["resident_instances"] == 1 and a.classes()["items"][1] > 0
+    u = torch.as_tensor(rng.normal(size=a.n_elems) + 1j * rng.normal(size=a.n_elems), device=a.device)
+    ra, rb = a.rhs(u), b.rhs(u)
+    assert (ra - rb).abs().max().item() <= 1e-12 * rb.abs().max().item()
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("herm", [False, True])
+def test_rhs_combine_matches_rhs_and_axpy(herm):
+    """pyqed_heom_rhs_combine against pyqed_heom_rhs followed by the same combinations in torch: out =
+    base + alpha r, acc started (acc_init) and accumulated, out omitted; on a projector plan and on a
+    Hermitian plan (generic kernel)."""
+    import torch
+    from pyqed_amd import _lib
+    from pyqed_amd.heom.engine import HeomEngine
+    rng = np.random.default_rng(11)
+    inst = _proj_instance(7, 3, rng)
+    if herm:
+        inst = dict(inst, Q=np.real(inst["Q"]).astype(complex), sdip=None,
+                    bath=(inst["bath"][0], np.conj(inst["bath"][0]), inst["bath"][2], np.real(inst["bath"][3]) + 0j,
+                          inst["bath"][4]))
+    eng = HeomEngine([inst], hermitian=herm, packed=False, resident=False)
+    dev, n = eng.device, eng.n_elems
+    L, st = eng.L, _lib.stream_ptr(torch)
+    rnd = lambda: torch.as_tensor(rng.normal(size=n) + 1j * rng.normal(size=n), device=dev)  # noqa: E731
+    x, base, acc0 = rnd(), rnd(), rnd()
+    field = None if herm else torch.as_tensor(rng.normal(size=(2, 6)), device=dev)
+    fp = None if field is None else field.data_ptr()
+    r = eng.rhs(x, field=field, step=1, col=1)
+    out, acc = torch.zeros_like(x), acc0.clone()
+    _lib.check(L.pyqed_heom_rhs_combine(eng.plan, x.data_ptr(), out.data_ptr(), base.data_ptr(), 0.37, acc.data_ptr(),
+                                        -1.5, 0, fp, 1, 1, st))
+    m = int(eng.nmax[0]) * 49
+    tol = lambda ref: 1e-14 * ref[:m].abs().max().item()  # noqa: E731
+    assert (out[:m] - (base[:m] + 0.37 * r[:m])).abs().max().item() <= tol(base)
+    assert (acc[:m] - (acc0[:m] - 1.5 * r[:m])).abs().max().item() <= tol(acc0)
+    acc2 = acc0.clone()
+    _lib.check(L.pyqed_heom_rhs_combine(eng.plan, x.data_ptr(), None, None, 0.0, acc2.data_ptr(), 2.0, 1, fp, 1, 1, st))
+    assert (acc2[:m] - 2.0 * r[:m]).abs().max().item() <= tol(r)
+    out2 = torch.zeros_like(x)
+    _lib.check(L.pyqed_heom_rhs_combine(eng.plan, x.data_ptr(), out2.data_ptr(), None, 0.0, None, 0.0, 0, fp, 1, 1, st))
+    assert (out2[:m] - r[:m]).abs().max().item() == 0.0
+    assert L.pyqed_heom_rhs_combine(eng.plan, x.data_ptr(), x.data_ptr(), None, 0.0, None, 0.0, 0, fp, 1, 1, st) != 0
+    eng.close()

@@ -105,3 +105,11 @@ def test_host_checks_reject_non_finite_rho0_and_fields():
         prepare_instance(inst)
     with pytest.raises(ValueError, match="pulse"):
         field_table(3, 0.1, lambda t: np.inf if t > 0.15 else 0.0)
+
+
+def test_rhs_combine_argument_checks_without_a_device():
+    """pyqed_heom_rhs_combine rejects a null plan or input before any device work."""
+    from pyqed_amd import _lib
+    L = _lib.lib()
+    assert L.pyqed_heom_rhs_combine(None, None, None, None, 0.0, None, 0.0, 0, None, 0, 0, None) == 1
+    assert b"null plan or input" in L.pyqed_last_error()
