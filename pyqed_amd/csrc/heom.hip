@@ -637,6 +637,10 @@ constexpr int kChunk = PQA_CHUNK;  // neighbour loads per lane issued together
 // the neighbour-row requests (both round trips overlap the commutator)
 #define PQA_RK4_HOIST 1
 #endif
+#ifndef PQA_ROWS_DMA
+// wide projector class: neighbour rows by LDS DMA (global_load_lds_dword) instead of register chunks
+#define PQA_ROWS_DMA 1
+#endif
 #ifndef PQA_PACK_HOIST
 // packed MFMA forms: 0 request the RK4 operands of an ADO's upper elements once its neighbour rows
 // have arrived; 1 request them right behind the first chunk of neighbour rows (they then arrive during
@@ -894,7 +898,54 @@ void heom_proj_kernel(StageParams p) {
           }
         }
       };
-      issue(0, 0);
+      // LDS-DMA form: every 64-dword group of the row (and column) buffer is one global_load_lds_dword
+      // (lane l fills dword l of the group; lanes on padding dwords read a harmless in-bounds dword),
+      // the last partial group goes through one register per lane.  All rows of the ADO are then in
+      // flight at once with no registers held, instead of kCh values per lane per round trip.
+      // (the wide class only: it removes that class's register spills and shortens the 24-site chain's
+      // step by 6%; the 16 x 16 MFMA and VALU classes ran 1-10% slower with it)
+      constexpr bool kDMA = PQA_ROWS_DMA && NT == 2 && std::is_same<TI, T>::value;
+      constexpr int kW = (int)(sizeof(cpx<T>) / 4);  // dwords per complex value
+      float tail[2] = {0.f, 0.f};
+      auto dma_rows = [&]() {
+        const int m = kW * rs;  // dwords per row (or column) in LDS, padding included
+        // floor(p / m) = (p * mmag) >> 24 for p < 2^24 / m (p < 64 * 4 * 33 here)
+        const uint64_t mmag = ((1ull << 24) + m - 1) / m;
+        const int total = nent * m;
+        const int J = total >> 6;
+        const float* src = (const float*)xi;
+        auto source = [&](int pass, int pdw) -> int64_t {
+          const int t = (int)(((uint64_t)pdw * mmag) >> 24), k = pdw - t * m;
+          const int jj = k / kW;
+          if (jj >= d) return 0;
+          const int s2 = w.st[t];
+          const int64_t e = (int64_t)w.nb[t] * d2 + (pass == 0 ? s2 * d + jj : jj * d + s2);
+          return e * kW + (k - jj * kW);
+        };
+#pragma unroll
+        for (int pass = 0; pass < (HERM ? 1 : 2); ++pass) {
+          float* dst = (float*)(pass == 0 ? w.R : w.Cc);
+#pragma unroll 4
+          for (int j = 0; j < J; ++j)
+            __builtin_amdgcn_global_load_lds(src + source(pass, 64 * j + lane),
+                                             (__attribute__((address_space(3))) void*)(dst + 64 * j), 4, 0, 0);
+          const int pdw = 64 * J + lane;
+          if (pdw < total) tail[pass] = src[source(pass, pdw)];
+        }
+      };
+      auto dma_finish = [&]() {
+        // the DMA writes are complete (vmcnt counts them); the asm's memory clobber keeps every LDS read
+        // of the buffers below it
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        const int total = nent * kW * rs;
+        const int pdw = 64 * (total >> 6) + lane;
+        if (pdw < total) {
+          ((float*)w.R)[pdw] = tail[0];
+          if (!HERM) ((float*)w.Cc)[pdw] = tail[1];
+        }
+      };
+      if constexpr (kDMA) dma_rows();
+      else issue(0, 0);
       cpx<T> acc[4 * NT * NT];  // MFMA layout: element (16 ti + 4 gk + r, 16 tj + gi) at (ti NT + tj) 4 + r
       Pre<kDelta> pr[NT == 1 ? 4 : kQP];
       PreM pm[NT == 1 ? 4 : kQP];
@@ -911,7 +962,7 @@ void heom_proj_kernel(StageParams p) {
           }
         }
       };
-      if (MFMA && PACK && !(kPM && STAGE == 0) && PQA_PACK_HOIST == 1) request_packed();
+      if (MFMA && PACK && !(kPM && STAGE == 0) && (PQA_PACK_HOIST == 1 || kDMA)) request_packed();
       if (MFMA && NT == 1 && !PACK && PQA_RK4_HOIST == 2) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -981,16 +1032,20 @@ void heom_proj_kernel(StageParams p) {
               if (i < d && j < d) cmsub(a2, damp, w.X[i * rs + j]);
             }
       }
-      stash(0, 0);
-      for (int pass = 0; pass < (HERM ? 1 : 2); ++pass)
-        for (int q0 = pass == 0 ? 64 * kCh : 0; q0 < nval; q0 += 64 * kCh) {
-          issue(pass, q0);
-          stash(pass, q0);
-        }
+      if constexpr (kDMA) {
+        dma_finish();
+      } else {
+        stash(0, 0);
+        for (int pass = 0; pass < (HERM ? 1 : 2); ++pass)
+          for (int q0 = pass == 0 ? 64 * kCh : 0; q0 < nval; q0 += 64 * kCh) {
+            issue(pass, q0);
+            stash(pass, q0);
+          }
+      }
       // RK4 operands of the lane's four elements, requested together (one round trip instead of
       // one per element); the gather below overlaps their latency
       if (MFMA && PACK && !(kPM && STAGE == 0)) {
-        if (PQA_PACK_HOIST == 0) request_packed();
+        if (PQA_PACK_HOIST == 0 && !kDMA) request_packed();
       } else if (MFMA && NT == 1 && PQA_RK4_HOIST == 1) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -3181,10 +3236,12 @@ __global__ __launch_bounds__(kThreads) void heom_adjoint_contract_kernel(AdjPara
   }
 }
 
-// G[c][traj_off + f] = sum over the instance's items of their partials.  Block (instance, x) covers 16
-// of the instance's 3 d^2 values; its 256 threads are 16 values x 16 item lanes, each lane summing
-// every 16th item, and the 16 lane sums are added in a fixed order (deterministic results).
-constexpr int kRedVals = 16, kRedLanes = kThreads / kRedVals;
+// G[c][traj_off + f] = sum over the instance's items of their partials.  Block (instance, x) covers 4
+// of the instance's 3 d^2 values; its 256 threads are 4 values x 64 item lanes.  Lane ln sums items
+// ln, ln + 64, ... into four accumulators (item k into accumulator (k / 64) mod 4, so four loads are
+// in flight), and the 64 lane sums are added by a fixed-shape tree: the order never depends on
+// timing, so results are deterministic.
+constexpr int kRedVals = 4, kRedLanes = kThreads / kRedVals;
 
 __global__ __launch_bounds__(kThreads) void heom_adjoint_reduce_kernel(AdjParams q) {
   __shared__ double2 red[kRedLanes][kRedVals];
@@ -3195,23 +3252,37 @@ __global__ __launch_bounds__(kThreads) void heom_adjoint_reduce_kernel(AdjParams
   const int e = blockIdx.x * kRedVals + v;
   const bool act = e < 3 * I.d2;
   const int c = act ? e / I.d2 : 0, f = act ? e - c * I.d2 : 0;
-  double2 sum = make_double2(0.0, 0.0);
-  if (act)
-    for (int it = r.x + ln; it < r.y; it += kRedLanes) {
+  double2 s4[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) s4[u] = make_double2(0.0, 0.0);
+  if (act) {
+    int it = r.x + ln;
+    for (; it + 3 * kRedLanes < r.y; it += 4 * kRedLanes) {
+      double2 w[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) w[u] = q.part[((int64_t)(it + u * kRedLanes) * 3 + c) * q.d2max + f];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        s4[u].x += w[u].x;
+        s4[u].y += w[u].y;
+      }
+    }
+    for (int u = 0; it < r.y; it += kRedLanes, ++u) {
       const double2 w = q.part[((int64_t)it * 3 + c) * q.d2max + f];
-      sum.x += w.x;
-      sum.y += w.y;
+      s4[u].x += w.x;
+      s4[u].y += w.y;
     }
-  red[ln][v] = sum;
-  __syncthreads();
-  if (ln == 0 && act) {
-    double2 t = red[0][v];
-    for (int k = 1; k < kRedLanes; ++k) {
-      t.x += red[k][v].x;
-      t.y += red[k][v].y;
-    }
-    q.G[c * q.traj_stride + I.traj_off + f] = t;
   }
+  red[ln][v] = make_double2((s4[0].x + s4[1].x) + (s4[2].x + s4[3].x), (s4[0].y + s4[1].y) + (s4[2].y + s4[3].y));
+  __syncthreads();
+  for (int h = kRedLanes / 2; h > 0; h >>= 1) {
+    if (ln < h) {
+      red[ln][v].x += red[ln + h][v].x;
+      red[ln][v].y += red[ln + h][v].y;
+    }
+    __syncthreads();
+  }
+  if (ln == 0 && act) q.G[c * q.traj_stride + I.traj_off + f] = red[0][v];
 }
 
 size_t adjoint_lds_bytes(const pyqed_heom_plan* P) {
