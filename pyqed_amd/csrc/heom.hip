@@ -801,6 +801,11 @@ void heom_proj_kernel(StageParams p) {
   const cpx_store<TI>* xin = (const cpx_store<TI>*)p.xin;
   cpx_store<TO>* xout = (cpx_store<TO>*)p.xout;
   const int col = p.col_override >= 0 ? p.col_override : stage_col(STAGE & 3);
+  // neighbour rows by LDS DMA in the wide class only: it removes that class's register spills and
+  // shortens the 24-site chain's step by 6%; the 16 x 16 MFMA and VALU classes ran 1-10% slower with it.
+  // Hermitian DMA plans keep, per entry, the row's index in the instance (neighbour * d + site) in
+  // place of the site.
+  constexpr bool kDMA = PQA_ROWS_DMA && NT == 2 && std::is_same<TI, T>::value;
   int it0, itstep, itend;
   item_range(p, it0, itend, itstep);
   for (int it = it0; it < itend; it += itstep) {
@@ -863,7 +868,7 @@ void heom_proj_kernel(StageParams p) {
         const cpx<T>* C = sC + 5 * k;
         const T sq = sS[nf];
         w.nb[pos] = en.x;
-        w.st[pos] = my_site;
+        w.st[pos] = (kDMA && HERM) ? en.x * d + my_site : my_site;
         w.cl[pos] = (plus ? C[3] : C[1]) * sq;
         w.cr[pos] = (plus ? C[4] : C[2]) * sq;
         if (!plus) dp = C[0] * (T)nf;
@@ -898,13 +903,11 @@ void heom_proj_kernel(StageParams p) {
           }
         }
       };
-      // LDS-DMA form: every 64-dword group of the row (and column) buffer is one global_load_lds_dword
-      // (lane l fills dword l of the group; lanes on padding dwords read a harmless in-bounds dword),
-      // the last partial group goes through one register per lane.  All rows of the ADO are then in
-      // flight at once with no registers held, instead of kCh values per lane per round trip.
-      // (the wide class only: it removes that class's register spills and shortens the 24-site chain's
-      // step by 6%; the 16 x 16 MFMA and VALU classes ran 1-10% slower with it)
-      constexpr bool kDMA = PQA_ROWS_DMA && NT == 2 && std::is_same<TI, T>::value;
+      // LDS-DMA form (kDMA): every 64-dword group of the row (and column) buffer is one
+      // global_load_lds_dword (lane l fills dword l of the group; lanes on padding dwords read a harmless
+      // in-bounds dword), the last partial group goes through one register per lane.  All rows of the
+      // ADO are then in flight at once with no registers held, instead of kCh values per lane per round
+      // trip.
       constexpr int kW = (int)(sizeof(cpx<T>) / 4);  // dwords per complex value
       float tail[2] = {0.f, 0.f};
       auto dma_rows = [&]() {
@@ -914,6 +917,29 @@ void heom_proj_kernel(StageParams p) {
         const int total = nent * m;
         const int J = total >> 6;
         const float* src = (const float*)xi;
+        if constexpr (HERM) {
+          // rows only: lane position (t, k) in the buffer advanced by 64 dwords per group (m >= 36, so at
+          // most two wraps), the row start read once per group from the per-entry row index
+          float* dst = (float*)w.R;
+          int t = lane / m, k = lane - (lane / m) * m;
+          const int rw = kW * d;  // dwords of a row
+#pragma unroll 4
+          for (int j = 0; j < J; ++j) {
+            const int64_t off = k < rw ? (int64_t)w.st[t] * rw + k : 0;
+            __builtin_amdgcn_global_load_lds(src + off, (__attribute__((address_space(3))) void*)(dst + 64 * j), 4, 0, 0);
+            k += 64;
+            if (k >= m) {
+              k -= m;
+              ++t;
+            }
+            if (k >= m) {
+              k -= m;
+              ++t;
+            }
+          }
+          if (64 * J + lane < total) tail[0] = src[k < rw ? (int64_t)w.st[t] * rw + k : 0];
+          return;
+        }
         auto source = [&](int pass, int pdw) -> int64_t {
           const int t = (int)(((uint64_t)pdw * mmag) >> 24), k = pdw - t * m;
           const int jj = k / kW;
