@@ -29,7 +29,7 @@ usage: python bench.py [--steps K] [--warmup W] [--workload c3|c2|c5|c1|c4|lindb
 
 Other workloads: c4 (Lindblad adjoint gradients), lindblad (driven Lindblad forward), legacy /
 legacy-rk4 (oqs.HEOMSolver), spectral (eigen-decomposition spectral sums), krylov (HEOM spectra),
-heom-adjoint (streamed HEOM gradients).
+heom-adjoint (streamed HEOM gradients), lindblad-matrix (Lindblad in matrix form at d = 64).
 """
 import argparse
 import json
@@ -659,6 +659,106 @@ def bench_lindblad(args, torch):
     print(json.dumps(out), flush=True)
 
 
+LINDBLAD_MATRIX_METRIC = ("instance-steps/s, Lindblad RK4 in matrix form (64-level damped cavity, decay and "
+                          "dephasing, FP64), 1 GPU")
+
+
+def cavity_problem(d=64, n=256, seed=21):
+    """A Kerr cavity truncated at d levels, H = w a^dag a + chi (a^dag a)^2, collapse operators sqrt(kappa) a
+    and sqrt(gamma) a^dag a; n seeded coherent initial states."""
+    a = np.diag(np.sqrt(np.arange(1, d)), 1).astype(complex)
+    num = a.conj().T @ a
+    H = 1.0 * num + 0.01 * num @ num
+    c_ops = [np.sqrt(0.05) * a, np.sqrt(0.02) * num]
+    rng = np.random.default_rng(seed)
+    rhos = []
+    for alpha in rng.uniform(1.0, 3.0, n) * np.exp(2j * np.pi * rng.uniform(size=n)):
+        v = np.zeros(d, complex)
+        v[0] = 1.0
+        for k in range(1, d):
+            v[k] = v[k - 1] * alpha / np.sqrt(k)
+        v /= np.linalg.norm(v)
+        rhos.append(np.outer(v, v.conj()))
+    return H, c_ops, np.array(rhos), [num, a + a.conj().T]
+
+
+def bench_lindblad_matrix(args, torch):
+    """LindbladSolver.run for a d = 64 system (beyond the Liouville kernel's d*d <= 1024): the matrix-form
+    kernel, n_inst initial states per launch, two observables every step (oqs.py:1596-1696)."""
+    from pyqed_amd.liouville import propagate_matrix
+    d, n, nsteps, dt = 64, args.n_inst if args.n_inst != 4096 else 256, 200, 0.01
+    H, c_ops, rhos, e_ops = cavity_problem(d, n)
+    for _ in range(args.warmup):
+        propagate_matrix(H, c_ops, [], rhos, dt, nsteps, None, e_ops, keep_states=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        obs, _, final = propagate_matrix(H, c_ops, [], rhos, dt, nsteps, None, e_ops, keep_states=False)
+    wall = (time.perf_counter() - t0) / args.steps
+    # kernel time alone (the host wrapper also uploads operators and states and downloads results)
+    from pyqed_amd import _lib
+    L = _lib.lib()
+    dev = torch.device("cuda", 0)
+    Heff = H - 0.5j * sum(c.conj().T @ c for c in c_ops)
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)  # noqa: E731
+    He, HeD = t(Heff), t(Heff.conj().T)
+    Cm, CDm = t(np.array(c_ops)), t(np.array([c.conj().T for c in c_ops]))
+    rho = t(rhos)
+    O = t(np.array([e.T.reshape(-1) for e in e_ops]))
+    obs_out = torch.empty((n, nsteps + 1, 2), dtype=torch.complex128, device=dev)
+    wb = _lib.C.c_int64()
+    _lib.check(L.pyqed_lindblad_work_bytes(d, 2, 0, n, _lib.C.byref(wb)))
+    work = torch.empty(wb.value, dtype=torch.uint8, device=dev)
+    st = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def launch():
+        _lib.check(L.pyqed_lindblad_run(d, 2, He.data_ptr(), HeD.data_ptr(), Cm.data_ptr(), CDm.data_ptr(), 0, None,
+                                        None, n, rho.data_ptr(), None, nsteps, dt, 2, O.data_ptr(), obs_out.data_ptr(),
+                                        None, 1, work.data_ptr(), work.numel(), st.cuda_stream))
+
+    launch()
+    ev[0].record(st)
+    launch()
+    ev[1].record(st)
+    torch.cuda.synchronize()
+    launch_ms = ev[0].elapsed_time(ev[1])
+    flops = 4 * (2 + 2 * len(c_ops)) * 8 * d ** 3 * nsteps * n  # complex products of an RK4 step
+    achieved = flops / (launch_ms / 1e3) / 1e12
+    trace_dev = float(np.abs(np.trace(final, axis1=1, axis2=2) - 1).max())
+    out = {
+        "metric": LINDBLAD_MATRIX_METRIC, "value": n * nsteps / wall, "unit": "instance-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (Kerr cavity, seeded coherent initial states)",
+        "config": {"workload": f"Kerr cavity truncated at d = {d} (w = 1, chi = 0.01), collapse operators "
+                               f"sqrt(0.05) a and sqrt(0.02) a^dag a, {n} initial states per launch, {nsteps} RK4 "
+                               f"steps (dt {dt}), <n> and <x> every step; a step is one call (upload, launch, "
+                               "download)", "instances": n, "d": d},
+        "roofline": {"bound": "mfma", "kernel": "lindblad_matrix_kernel", "launch_ms": launch_ms,
+                     "flops_per_launch": flops, "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "note": "FP64 VALU FMAs; the MI355X FP64 vector and matrix peaks are equal"},
+        "trace_deviation": trace_dev,
+    }
+    if not args.no_cpu:
+        print("[bench] cpu baseline leg", file=sys.stderr, flush=True)
+        from oracle import lindblad as OL
+        import threadpoolctl
+        info = threadpoolctl.threadpool_info()
+        nthr = max([i.get("num_threads", 1) for i in info] or [1])
+        model, host = cpu_info()
+        k = 20
+        t0 = time.perf_counter()
+        OL.run(H, c_ops, rhos[0], dt, k)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": k / el, "unit": "instance-steps/s", "cores": nthr, "kind": "port",
+                               "cpu_model": model, "host_cores": host,
+                               "sample": f"oracle.lindblad.run (numpy FP64 dense products, BLAS threads {nthr}): "
+                                         f"{k} RK4 steps of one instance in {el:.2f} s"}
+    print(json.dumps(out), flush=True)
+
+
 SPECTRAL_METRIC = "Liouville-propagator entries/s, eigen-expansion U(t) = U1 e^{lambda t} U1^-1 (FP64), 1 GPU"
 
 
@@ -1117,7 +1217,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; 2000 for --workload legacy)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "lindblad", "legacy",
-                                                              "legacy-rk4", "spectral", "krylov", "heom-adjoint"])
+                                                              "legacy-rk4", "spectral", "krylov", "heom-adjoint", "lindblad-matrix"])
     ap.add_argument("--n-inst", type=int, default=None, help="instances (default 4096; 65536 for legacy)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--state", default="auto", choices=["auto", "full"],
@@ -1145,7 +1245,7 @@ def main():
     args = ap.parse_args()
     legacy = args.workload in ("legacy", "legacy-rk4")
     if args.steps is None:
-        args.steps = 2000 if legacy else (3 if args.workload in ("c4", "lindblad", "heom-adjoint") else 20)
+        args.steps = 2000 if legacy else (3 if args.workload in ("c4", "lindblad", "heom-adjoint", "lindblad-matrix") else 20)
     if args.n_inst is None:
         args.n_inst = 65536 if legacy else 4096
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -1176,6 +1276,8 @@ def main():
     if args.workload == "c4":
         bench_c4(args, torch)
         return
+    if args.workload == "lindblad-matrix":
+        return bench_lindblad_matrix(args, torch)
     if args.workload == "lindblad":
         bench_lindblad(args, torch)
         return

@@ -231,6 +231,30 @@ int pyqed_liouville_run(int32_t N, int32_t nterm, const void* A, int32_t n_inst,
                         const double* field, int64_t nsteps, double dt, int32_t n_obs, const void* obs,
                         void* obs_out, void* rho_out, int64_t save_every, void* stream);
 
+/* ------------------------------------------------------------------ Lindblad, matrix form
+ * Batched propagation of rho (d x d, row-major) under the Lindblad equation for any d (the
+ * Liouville-space entry point above holds the N x N generator and serves N = d*d <= 1024):
+ *   d rho/dt = -i (Hs rho - rho Hs^dag) + sum_k c_k rho c_k^dag,  Hs = Heff - sum_i f_i(step) H_i,
+ * Heff = H0 - (i/2) sum_k c_k^dag c_k (formed by the caller), i.e. the reference's liouvillian
+ * (pyqed/oqs.py:697-714) with the anticommutators folded into Heff, stepped by the RK4 of
+ * pyqed/phys.py:1051-1063 with the drive held over each step.  Replaces the time loops of _lindblad
+ * and _lindblad_driven (oqs.py:1596-1806) for large d.
+ *   Heff, HeffD  device [d][d] complex double: Heff and its adjoint
+ *   C, CD        device [nc][d][d]: c_k and c_k^dag (NULL when nc == 0)
+ *   Hd, HdD      device [ndrive][d][d]: drive operators H_i and their adjoints (NULL when ndrive == 0)
+ *   rho          device [n_inst][d][d], advanced in place
+ *   field        device [n_inst][nsteps][ndrive] double (NULL when ndrive == 0)
+ *   obs          device [n_obs][d][d], O^T flattened (Tr(O rho) = sum_e obs[e] rho[e]); obs_out
+ *                device [n_inst][nsteps+1][n_obs] (row 0 = the initial state)
+ *   rho_out      device [n_inst][nsteps/save_every][d][d] or NULL
+ *   work         device, >= pyqed_lindblad_work_bytes(d, nc, ndrive, n_inst) bytes */
+int pyqed_lindblad_work_bytes(int32_t d, int32_t nc, int32_t ndrive, int32_t n_inst, int64_t* bytes);
+int pyqed_lindblad_run(int32_t d, int32_t nc, const void* Heff, const void* HeffD, const void* C, const void* CD,
+                       int32_t ndrive, const void* Hd, const void* HdD, int32_t n_inst, void* rho,
+                       const double* field, int64_t nsteps, double dt, int32_t n_obs, const void* obs,
+                       void* obs_out, void* rho_out, int64_t save_every, void* work, int64_t work_bytes,
+                       void* stream);
+
 /* Adjoint of pyqed_liouville_run for a real objective J(rho(T)) (no
  * reference counterpart: pyqed has no gradients).  Forward states are
  * checkpointed every `ckpt_every` steps and each segment is recomputed

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB = os.path.join(HERE, "libpyqed_amd.so")
-SOURCES = ["capi.cpp", "heom.hip", "liouville.hip", "legacy.hip", "spectral.hip"]
+SOURCES = ["capi.cpp", "heom.hip", "liouville.hip", "lindblad.hip", "legacy.hip", "spectral.hip"]
 ARCH = os.environ.get("PYQED_AMD_ARCH", "gfx950")
 
 

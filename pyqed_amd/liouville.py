@@ -67,3 +67,52 @@ def propagate(gens, rho0s, dt, nsteps, field=None, e_ops=(), save_every=1, keep_
     o = obs_out.cpu().numpy() if n_obs else np.zeros((n_inst, nsteps + 1, 0), complex)
     st = rho_out.cpu().numpy().reshape(n_inst, -1, d, d)[:, :nsave] if nsave else np.zeros((n_inst, 0, d, d), complex)
     return o, st, rho.cpu().numpy().reshape(n_inst, d, d)
+
+
+def propagate_matrix(H0, c_ops, drives, rho0s, dt, nsteps, field=None, e_ops=(), save_every=1, keep_states=True):
+    """Run pyqed_lindblad_run: the Lindblad equation in matrix form (any d), with H(t) = H0 -
+    sum_i f_i H_i for drives H_i and field [n_inst, nsteps, len(drives)].  Returns the same
+    (observables [n_inst, nsteps+1, n_obs], states [n_inst, nsave, d, d], final [n_inst, d, d]) as
+    propagate."""
+    torch = _lib.require_gpu()
+    L = _lib.lib()
+    H0 = np.asarray(H0, np.complex128)
+    d = H0.shape[0]
+    cs = [np.asarray(c, np.complex128) for c in (c_ops or [])]
+    Heff = H0 - 0.5j * sum((dag(c) @ c for c in cs), np.zeros_like(H0))
+    hs = [np.asarray(h, np.complex128) for h in (drives or [])]
+    rho0s = np.asarray(rho0s, np.complex128)
+    n_inst = rho0s.shape[0]
+    dev = torch.device("cuda", torch.cuda.current_device())
+
+    def dev_arr(a):
+        return torch.as_tensor(np.ascontiguousarray(np.asarray(a, np.complex128)), device=dev)
+
+    He, HeD = dev_arr(Heff), dev_arr(dag(Heff))
+    Cm = dev_arr(np.array(cs)) if cs else None
+    CDm = dev_arr(np.array([dag(c) for c in cs])) if cs else None
+    Hd = dev_arr(np.array(hs)) if hs else None
+    HdD = dev_arr(np.array([dag(h) for h in hs])) if hs else None
+    f = None
+    if hs:
+        f = torch.as_tensor(np.ascontiguousarray(np.asarray(field, np.float64).reshape(n_inst, nsteps, len(hs))),
+                            device=dev)
+    rho = dev_arr(rho0s.reshape(n_inst, d, d)).clone()
+    n_obs = len(e_ops)
+    obs = obs_out = None
+    if n_obs:
+        obs = dev_arr(np.array([np.asarray(e, np.complex128).T.reshape(-1) for e in e_ops]))
+        obs_out = torch.empty((n_inst, nsteps + 1, n_obs), dtype=torch.complex128, device=dev)
+    nsave = nsteps // save_every if keep_states else 0
+    rho_out = torch.empty((n_inst, max(nsave, 1), d * d), dtype=torch.complex128, device=dev) if nsave else None
+    wb = _lib.C.c_int64()
+    _lib.check(L.pyqed_lindblad_work_bytes(d, len(cs), len(hs), n_inst, _lib.C.byref(wb)))
+    work = torch.empty(max(wb.value, 16), dtype=torch.uint8, device=dev)
+    _lib.check(L.pyqed_lindblad_run(d, len(cs), He.data_ptr(), HeD.data_ptr(), _lib.ptr(Cm), _lib.ptr(CDm), len(hs),
+                                    _lib.ptr(Hd), _lib.ptr(HdD), n_inst, rho.data_ptr(), _lib.ptr(f), nsteps, float(dt),
+                                    n_obs, _lib.ptr(obs), _lib.ptr(obs_out), _lib.ptr(rho_out), save_every,
+                                    work.data_ptr(), work.numel(), _lib.stream_ptr(torch)))
+    torch.cuda.synchronize()
+    o = obs_out.cpu().numpy() if n_obs else np.zeros((n_inst, nsteps + 1, 0), complex)
+    st = rho_out.cpu().numpy().reshape(n_inst, -1, d, d)[:, :nsave] if nsave else np.zeros((n_inst, 0, d, d), complex)
+    return o, st, rho.cpu().numpy().reshape(n_inst, d, d)

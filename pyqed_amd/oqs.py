@@ -18,16 +18,23 @@ import numpy as np
 import scipy.linalg
 
 from . import _lib
-from .liouville import dag, lindblad_generator, op2sop, propagate
+from .liouville import dag, lindblad_generator, op2sop, propagate, propagate_matrix
 from .mol import Result
 from .spectral import TIME, spectral_sum
 
 
 class LindbladSolver:
-    def __init__(self, H=None, c_ops=None, e_ops=None):
+    """form (keyword only): "auto" runs d*d <= 1024 on the Liouville-space kernel (vec(rho) and its
+    generator in LDS) and larger d on the matrix-form kernel (pyqed_lindblad_run); "liouville" or
+    "matrix" force one of them."""
+
+    def __init__(self, H=None, c_ops=None, e_ops=None, *, form="auto"):
+        if form not in ("auto", "liouville", "matrix"):
+            raise ValueError("form must be 'auto', 'liouville' or 'matrix'")
         self.c_ops = c_ops
         self.e_ops = e_ops
         self.H = H
+        self.form = form
 
     def set_c_ops(self, c_ops):
         self.c_ops = c_ops
@@ -46,15 +53,32 @@ class LindbladSolver:
         """Liouvillian superoperator (superoperator.liouvillian), row-major vec."""
         return lindblad_generator(self.H, self.c_ops or [])
 
+    def _prop(self, rho0s, dt, Nt, field=None, e_ops=(), keep_states=True, static=False):
+        """Propagate initial states [n, d, d] (field [n, Nt, n_drive] for a driven H) on the kernel that
+        `form` selects; returns (observables, states, final) as liouville.propagate."""
+        c_ops = self.c_ops or []
+        rho0s = np.asarray(rho0s, np.complex128)
+        d = rho0s.shape[-1]
+        if isinstance(self.H, list):
+            if static:
+                raise NotImplementedError("correlation functions need a static Hamiltonian")
+            H0 = np.asarray(self.H[0], np.complex128)
+            drives = [np.asarray(Hi, np.complex128) for Hi, _ in self.H[1:]]
+        else:
+            H0 = np.asarray(self.H, np.complex128)
+            drives = []
+        if self.form == "matrix" or (self.form == "auto" and d * d > 1024):
+            return propagate_matrix(H0, c_ops, drives, rho0s, dt, Nt, field if drives else None, e_ops,
+                                    keep_states=keep_states)
+        gens = [lindblad_generator(H0, c_ops)] + [1j * op2sop(Hi) for Hi in drives]
+        return propagate(np.array(gens), rho0s, dt, Nt, field if drives else None, e_ops, keep_states=keep_states)
+
     def run(self, rho0, dt, Nt, t0=0.0, e_ops=None, return_result=True, *, legacy_accumulate=False):
         e_ops = [] if e_ops is None else list(e_ops)
-        c_ops = self.c_ops or []
         rho0 = np.asarray(rho0, np.complex128)
         result = Result(dt=dt, Nt=Nt, rho0=rho0)
         if isinstance(self.H, list):
-            H0 = np.asarray(self.H[0], np.complex128)
             terms = self.H[1:]
-            gens = [lindblad_generator(H0, c_ops)] + [1j * op2sop(np.asarray(Hi, np.complex128)) for Hi, _ in terms]
             field = np.zeros((Nt, len(terms)))
             for k in range(Nt):
                 t = t0 + (k + 1) * dt
@@ -65,11 +89,10 @@ class LindbladSolver:
                     field[k, i] = v.real
             if legacy_accumulate:
                 field = np.cumsum(field, axis=0)
-            obs, states, _ = propagate(np.array(gens), rho0[None], dt, Nt, field[None], e_ops)
+            obs, states, _ = self._prop(rho0[None], dt, Nt, field[None], e_ops)
             result.observables = obs[0, 1:]
         else:
-            gens = lindblad_generator(np.asarray(self.H, np.complex128), c_ops)[None]
-            obs, states, _ = propagate(gens, rho0[None], dt, Nt, None, e_ops)
+            obs, states, _ = self._prop(rho0[None], dt, Nt, None, e_ops)
             result.observables = obs[0]
         result.rholist = [states[0, k] for k in range(states.shape[1])]
         return result
@@ -77,31 +100,19 @@ class LindbladSolver:
     def run_batch(self, rho0s, dt, Nt, fields=None):
         """Propagate many initial states (and, for a driven H, per-instance fields [n, Nt, n_terms])
         in one launch; returns the final states [n, d, d]."""
-        c_ops = self.c_ops or []
-        rho0s = np.asarray(rho0s, np.complex128)
-        if isinstance(self.H, list):
-            H0 = np.asarray(self.H[0], np.complex128)
-            gens = [lindblad_generator(H0, c_ops)] + [1j * op2sop(np.asarray(Hi, np.complex128)) for Hi, _ in self.H[1:]]
-        else:
-            gens = [lindblad_generator(np.asarray(self.H, np.complex128), c_ops)]
-        _, _, final = propagate(np.array(gens), rho0s, dt, Nt, fields, (), keep_states=False)
+        _, _, final = self._prop(rho0s, dt, Nt, fields, (), keep_states=False)
         return final
 
     # ---------------------------------------------------------------- correlation functions
     # Quantum regression on the GPU Liouville kernel (oqs.py:717-791, 1193-1327).  The propagations
     # of a two-time correlation (one per t) run as one batched launch.
 
-    def _static_generator(self):
-        if isinstance(self.H, list):
-            raise NotImplementedError("correlation functions need a static Hamiltonian")
-        return lindblad_generator(np.asarray(self.H, np.complex128), self.c_ops or [])[None]
-
     def correlation_2op_1t(self, rho0, a_op, b_op, dt, Nt, output='cor.dat'):
         """<A(t)B> = Tr[A e^{Lt}(B rho0)] at t = dt, ..., Nt*dt (oqs.py:1193-1222 -> 717-791): Nt
         values.  As the reference, lines 't value' are written to `output` (None: no file)."""
         A = np.asarray(a_op, np.complex128)
         start = np.asarray(b_op, np.complex128) @ np.asarray(rho0, np.complex128)
-        obs, _, _ = propagate(self._static_generator(), start[None], dt, Nt, None, [A], keep_states=False)
+        obs, _, _ = self._prop(start[None], dt, Nt, None, [A], keep_states=False, static=True)
         cor = obs[0, 1:, 0].copy()
         if output is not None:
             with open(output, 'w') as f:
@@ -115,7 +126,7 @@ class LindbladSolver:
         """<A B(t) C> = Tr[B e^{Lt}(C rho0 A)] at t = 0, dt, ..., Nt*dt (oqs.py:1224-1242): Nt+1 values."""
         a_op, b_op, c_op = (np.asarray(o, np.complex128) for o in oplist)
         start = c_op @ np.asarray(rho0, np.complex128) @ a_op
-        obs, _, _ = propagate(self._static_generator(), start[None], dt, Nt, None, [b_op], keep_states=False)
+        obs, _, _ = self._prop(start[None], dt, Nt, None, [b_op], keep_states=False, static=True)
         return obs[0, :, 0]
 
     def correlation_4op_1t(self, rho0, ops, dt, nt):
@@ -132,10 +143,9 @@ class LindbladSolver:
         one launch.  (The reference stores the Ntau+1 values of each propagation in a row of width
         Ntau, which raises; the first Ntau values, tau = 0 .. (Ntau-1) dt, are kept here.)"""
         a_op, b_op, c_op = (np.asarray(o, np.complex128) for o in ops)
-        gen = self._static_generator()
-        _, states, _ = propagate(gen, np.asarray(rho0, np.complex128)[None], dt, Nt, None, ())
+        _, states, _ = self._prop(np.asarray(rho0, np.complex128)[None], dt, Nt, None, (), static=True)
         starts = c_op @ states[0] @ a_op
-        obs, _, _ = propagate(gen, starts, dt, Ntau, None, [b_op], keep_states=False)
+        obs, _, _ = self._prop(starts, dt, Ntau, None, [b_op], keep_states=False, static=True)
         return obs[:, :Ntau, 0]
 
     def correlation_4op_2t(self, rho0, ops, dt, nt, ntau):
