@@ -240,7 +240,9 @@ int pyqed_liouville_run(int32_t N, int32_t nterm, const void* A, int32_t n_inst,
  * pyqed/phys.py:1051-1063 with the drive held over each step.  Replaces the time loops of _lindblad
  * and _lindblad_driven (oqs.py:1596-1806) for large d.
  *   Heff, HeffD  device [d][d] complex double: Heff and its adjoint
- *   C, CD        device [nc][d][d]: c_k and c_k^dag (NULL when nc == 0)
+ *   C, CD        device [nc][d][d]: the left and right operators of the terms C_k rho CD_k (Lindblad:
+ *                c_k and c_k^dag; the Bloch-Redfield tensor of oqs.py:519-570 in operator form: pairs
+ *                (A_a, L_a^dag) and (L_a, A_a), with Heff = diag(E) - i sum_a A_a L_a); NULL when nc == 0
  *   Hd, HdD      device [ndrive][d][d]: drive operators H_i and their adjoints (NULL when ndrive == 0)
  *   rho          device [n_inst][d][d], advanced in place
  *   field        device [n_inst][nsteps][ndrive] double (NULL when ndrive == 0)

@@ -18,7 +18,7 @@ import numpy as np
 import scipy.linalg
 
 from . import _lib
-from .liouville import dag, lindblad_generator, op2sop, propagate, propagate_matrix
+from .liouville import dag, lindblad_generator, op2sop, propagate, propagate_matrix, redfield_operators, run_matrix
 from .mol import Result
 from .spectral import TIME, spectral_sum
 
@@ -179,7 +179,14 @@ def redfield_tensor(H, a_ops, spectra, secular=False):
 
 
 class RedfieldSolver:
-    def __init__(self, H, c_ops=None, spectra=None, e_ops=None):
+    """form (keyword only): "auto" runs evolve for d*d <= 1024 on the Liouville-space kernel with the
+    dense Redfield tensor and larger d on the matrix-form kernel with the tensor's operator form
+    (liouville.redfield_operators); "liouville" or "matrix" force one of them."""
+
+    def __init__(self, H, c_ops=None, spectra=None, e_ops=None, *, form="auto"):
+        if form not in ("auto", "liouville", "matrix"):
+            raise ValueError("form must be 'auto', 'liouville' or 'matrix'")
+        self.form = form
         self.H = H
         self.c_ops = c_ops
         self.R = None
@@ -288,13 +295,22 @@ class RedfieldSolver:
     def evolve(self, rho0, dt, Nt, evecs=None, e_ops=[], store_states=False, t0=0, nout=1):
         """RK4 on vec(rho) in the eigenbasis (oqs.py:57-81, 364-459): observables [Nt, n_e] in the
         eigenbasis (no t0 row), rholist back-transformed to the original basis."""
-        if self.R is None:
-            self.redfield_tensor()
-        ev = self.evecs
         rho0 = np.asarray(rho0, np.complex128)
-        e_eig = [dag(ev) @ np.asarray(e, np.complex128) @ ev for e in (e_ops or [])]
-        obs, states, _ = propagate(np.asarray(self.R, np.complex128)[None], (dag(ev) @ rho0 @ ev)[None], dt, Nt,
-                                   None, e_eig)
+        d = rho0.shape[-1]
+        if self.form == "matrix" or (self.form == "auto" and d * d > 1024):
+            if self.spectra is None:
+                raise TypeError("Specify the bath spectral function.")
+            Heff, lefts, rights, ev = redfield_operators(self.H, self.c_ops, self.spectra)
+            self.evecs = ev
+            e_eig = [dag(ev) @ np.asarray(e, np.complex128) @ ev for e in (e_ops or [])]
+            obs, states, _ = run_matrix(Heff, lefts, rights, (), (dag(ev) @ rho0 @ ev)[None], dt, Nt, None, e_eig)
+        else:
+            if self.R is None:
+                self.redfield_tensor()
+            ev = self.evecs
+            e_eig = [dag(ev) @ np.asarray(e, np.complex128) @ ev for e in (e_ops or [])]
+            obs, states, _ = propagate(np.asarray(self.R, np.complex128)[None], (dag(ev) @ rho0 @ ev)[None], dt, Nt,
+                                       None, e_eig)
         result = Result(dt=dt, Nt=Nt, rho0=rho0)
         result.observables = obs[0, 1:]
         result.rholist = [ev @ states[0, k] @ dag(ev) for k in range(states.shape[1])]

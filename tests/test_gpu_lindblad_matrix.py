@@ -103,3 +103,36 @@ def test_unitary_limit_keeps_trace_and_purity():
     assert abs(np.trace(rT) - 1) < 1e-12
     assert abs(np.trace(rT @ rT).real - np.trace(ref @ ref).real) < 1e-12
     assert abs(np.trace(rT @ rT).real - 1) < 1e-8
+
+
+def test_redfield_beyond_the_liouville_kernel():
+    """RedfieldSolver.evolve at d = 40 (N = 1600 > 1024) on the matrix-form kernel with the tensor's
+    operator form, against the oracle's Liouville-space RK4 with the dense tensor."""
+    from pyqed_amd.oqs import RedfieldSolver
+    rng = np.random.default_rng(12)
+    d = 40
+    H = np.diag(np.sort(rng.uniform(0, 2, d))) + 0.05 * _rand_herm(d, rng)
+    a_ops = [_rand_herm(d, rng) / np.sqrt(d), np.diag(rng.normal(size=d)).astype(complex)]
+    S = lambda w: 0.05 * w / (1 - np.exp(-2.0 * w)) if w != 0 else 0.025  # noqa: E731
+    rho0 = np.diag(rng.dirichlet(np.ones(d))).astype(complex)
+    e_ops = [np.diag(np.eye(d)[j]).astype(complex) for j in (0, 7, d - 1)]
+    res = RedfieldSolver(H, a_ops, [S, S]).evolve(rho0, 0.05, 20, e_ops=e_ops)
+    R, ev = OL.redfield_tensor(H, a_ops, [S, S])
+    ref = OL.run_redfield(R, rho0, ev, 0.05, 20, e_ops)
+    assert np.abs(res.observables - ref["observables"]).max() < 1e-12
+    assert np.abs(np.array(res.rholist) - np.array(ref["rholist"])).max() < 1e-12
+
+
+def test_redfield_matrix_form_equals_liouville_form():
+    from pyqed_amd.oqs import RedfieldSolver
+    rng = np.random.default_rng(13)
+    d = 4
+    H = np.diag([0.0, 0.5, 1.1, 1.9]) + 0.1 * _rand_herm(d, rng)
+    a_ops = [_rand_herm(d, rng), np.diag([1.0, -1, 0.5, 0]).astype(complex)]
+    S = lambda w: 0.05 * w / (1 - np.exp(-2.0 * w)) if w != 0 else 0.025  # noqa: E731
+    rho0 = np.diag([0.1, 0.2, 0.3, 0.4]).astype(complex)
+    e_ops = [np.diag(np.eye(d)[j]).astype(complex) for j in range(d)]
+    a = RedfieldSolver(H, a_ops, [S, S], form="matrix").evolve(rho0, 0.05, 300, e_ops=e_ops)
+    b = RedfieldSolver(H, a_ops, [S, S], form="liouville").evolve(rho0, 0.05, 300, e_ops=e_ops)
+    assert np.abs(a.observables - b.observables).max() < 1e-12
+    assert np.abs(np.array(a.rholist) - np.array(b.rholist)).max() < 1e-12
