@@ -686,7 +686,7 @@ def bench_lindblad_matrix(args, torch):
     """LindbladSolver.run for a d = 64 system (beyond the Liouville kernel's d*d <= 1024): the matrix-form
     kernel, n_inst initial states per launch, two observables every step (oqs.py:1596-1696)."""
     from pyqed_amd.liouville import propagate_matrix
-    d, n, nsteps, dt = 64, args.n_inst if args.n_inst != 4096 else 256, 200, 0.01
+    d, n, nsteps, dt = 64, args.n_inst if args.n_inst != 4096 else 1024, 200, 0.01
     H, c_ops, rhos, e_ops = cavity_problem(d, n)
     for _ in range(args.warmup):
         propagate_matrix(H, c_ops, [], rhos, dt, nsteps, None, e_ops, keep_states=False)
