@@ -278,7 +278,15 @@ def measured_traffic(name, key, workload, packed, tiling):
         t = json.load(f)
     if t.get("workload") != workload or bool(t.get("packed", False)) != bool(packed):
         return None
-    if t.get("tiling") != {k: tiling[k] for k in ("target_elems", "blocks_per_cu", "variant")}:
+    want = {k: tiling[k] for k in ("target_elems", "blocks_per_cu", "variant")}
+    if t.get("tiling") != want:
+        return None
+    # per-class tilings: those of the summary's run, or (older summaries) every class at the plan's
+    cls = tiling.get("classes") or []
+    if t.get("classes") is not None:
+        if t["classes"] != cls:
+            return None
+    elif any(c != [want["target_elems"], want["blocks_per_cu"]] for c in cls):
         return None
     k = t["per_dispatch"].get(name)
     return None if k is None else k["fetch_bytes"] + k["write_bytes"]
@@ -331,6 +339,7 @@ def traffic_from(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, bench_jso
     fk, wk = per_kernel(fetch_dir), per_kernel(write_dir)
     t = {"workload": b["config"]["workload"], "packed": b["state_layout"].startswith("packed"),
          "tiling": {k: b["tiling"][k] for k in ("target_elems", "blocks_per_cu", "variant")},
+         "classes": b["tiling"].get("classes"),
          "fetch_scale": fs, "write_scale": ws,
          "per_dispatch": {k: {"fetch_bytes": fk[k] * 1024.0 * fs, "write_bytes": wk.get(k, 0.0) * 1024.0 * ws}
                           for k in fk}}

@@ -176,6 +176,11 @@ int pyqed_heom_plan_stats(const pyqed_heom_plan* plan, int64_t* stats4);
  * set_tiling synchronises the device before replacing the queue. */
 int pyqed_heom_plan_set_tiling(pyqed_heom_plan* plan, int64_t target_elems, int32_t blocks_per_cu);
 int pyqed_heom_plan_get_tiling(const pyqed_heom_plan* plan, int64_t* target_elems, int32_t* blocks_per_cu);
+/* Work-item size and blocks per CU of one streamed kernel class (0 generic, 1 projector VALU, 2 projector
+ * MFMA 16 x 16, 3 projector MFMA 2 x 2), for the autotuner's per-class choice; set_tiling sets every
+ * class (no counterpart in the reference).  get_class_tiling fills host arrays [4]. */
+int pyqed_heom_plan_set_class_tiling(pyqed_heom_plan* plan, int32_t cls, int64_t target_elems, int32_t blocks_per_cu);
+int pyqed_heom_plan_get_class_tiling(const pyqed_heom_plan* plan, int64_t* target_elems, int32_t* blocks_per_cu);
 /* Kernel variants, chosen by the autotuner; results are identical.  Bit 0: the MFMA class (12 <= d <= 16)
  * of Hermitian full-matrix FP32 runs uses the two-deep software-pipelined kernel instead of one ADO per
  * wavefront at a time.  Bit 1: the wide class (17 <= d <= 32) of packed FP32 runs uses the

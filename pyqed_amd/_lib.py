@@ -24,7 +24,7 @@ PLAN_ADJOINT = 32
 SYMBOLS = [
     "pyqed_last_error", "pyqed_version", "pyqed_device_count",
     "pyqed_heom_plan_create", "pyqed_heom_plan_destroy", "pyqed_heom_plan_query", "pyqed_heom_plan_layout",
-    "pyqed_heom_tables_export", "pyqed_heom_tables_size", "pyqed_heom_tables_build", "pyqed_heom_run", "pyqed_heom_rhs", "pyqed_heom_rhs_combine", "pyqed_lindblad_work_bytes", "pyqed_lindblad_run", "pyqed_heom_plan_stats",
+    "pyqed_heom_tables_export", "pyqed_heom_tables_size", "pyqed_heom_tables_build", "pyqed_heom_run", "pyqed_heom_rhs", "pyqed_heom_rhs_combine", "pyqed_heom_plan_set_class_tiling", "pyqed_heom_plan_get_class_tiling", "pyqed_lindblad_work_bytes", "pyqed_lindblad_run", "pyqed_heom_plan_stats",
     "pyqed_heom_plan_classes", "pyqed_heom_plan_set_tiling", "pyqed_heom_plan_get_tiling", "pyqed_heom_plan_set_variant",
     "pyqed_heom_plan_class_state", "pyqed_heom_plan_set_timing", "pyqed_heom_plan_read_timing",
     "pyqed_heom_adjoint_work_bytes", "pyqed_heom_adjoint_contract",
@@ -78,6 +78,8 @@ def lib():
     L.pyqed_heom_rhs.argtypes = [vp, vp, vp, vp, i64, i32, vp]
     L.pyqed_heom_rhs_combine.argtypes = [vp, vp, vp, vp, C.c_double, vp, C.c_double, i32, vp, i64, i32, vp]
     L.pyqed_lindblad_work_bytes.argtypes = [i32, i32, i32, i32, C.POINTER(i64)]
+    L.pyqed_heom_plan_set_class_tiling.argtypes = [vp, i32, i64, i32]
+    L.pyqed_heom_plan_get_class_tiling.argtypes = [vp, vp, vp]
     L.pyqed_lindblad_run.argtypes = [i32, i32, vp, vp, vp, vp, i32, vp, vp, i32, vp, vp, i64, C.c_double, i32, vp, vp,
                                      vp, i64, vp, i64, vp]
     L.pyqed_heom_plan_stats.argtypes = [vp, vp]

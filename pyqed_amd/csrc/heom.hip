@@ -2254,6 +2254,8 @@ struct pyqed_heom_plan {
   std::vector<size_t> res_extra;  // per instance: resident_extra_bytes
   int64_t tile_target = 0;
   int blocks_per_cu = 8;
+  int64_t cls_target[kClasses] = {0, 0, 0, 0};  // per streamed class: work-item size and blocks per CU
+  int cls_bpc[kClasses] = {8, 8, 8, 8};
   int variant = 0;  // bit 0: pipelined kernel for the MFMA class (Hermitian full-matrix FP32); bit 1: block
                     // kernel for the wide class (packed FP32)
   int rowbuf = 1, colbuf = 0;  // projector path: max entries*d over instances (columns: non-Hermitian)
@@ -2385,15 +2387,27 @@ std::complex<double> cz(const double* p, int i) { return std::complex<double>(p[
 // classes first (generic, projector/VALU, projector/MFMA), resident instances last, and split each
 // class into eight contiguous ranges of equal element count (one per XCD).  `blocks_per_cu` sets the
 // persistent grid of the streamed kernels.
+void build_items(pyqed_heom_plan* P);
+
 void retile(pyqed_heom_plan* P, int64_t target, int blocks_per_cu) {
   if (target <= 0)
     target = std::min<int64_t>(8192, std::max<int64_t>(64, P->stream_elems / std::max<int64_t>(1, (int64_t)P->num_cu * 8)));
   P->tile_target = target;
   P->blocks_per_cu = std::max(1, std::min(blocks_per_cu, 32));
+  for (int c = 0; c < kClasses; ++c) {
+    P->cls_target[c] = P->tile_target;
+    P->cls_bpc[c] = P->blocks_per_cu;
+  }
+  build_items(P);
+}
+
+// The work items of every class at that class's item size; resident instances at the plan's.
+void build_items(pyqed_heom_plan* P) {
   P->items.clear();
   auto tile = [&](int i) {
     const InstDev& I = P->inst[i];
     const int nslot = (I.path == 0 || P->is_res[i]) ? kThreads / I.grp : kThreads / 64;
+    const int64_t target = P->is_res[i] ? P->tile_target : P->cls_target[I.path];
     int64_t per = (target + I.d2 - 1) / I.d2;
     per = ((per + nslot - 1) / nslot) * nslot;
     per = std::max<int64_t>(per, nslot);
@@ -2814,6 +2828,27 @@ extern "C" int pyqed_heom_plan_set_tiling(pyqed_heom_plan* P, int64_t target_ele
   return upload_items(P);
 }
 
+extern "C" int pyqed_heom_plan_set_class_tiling(pyqed_heom_plan* P, int32_t cls, int64_t target_elems,
+                                               int32_t blocks_per_cu) {
+  if (!P || cls < 0 || cls >= kClasses || target_elems < 1 || blocks_per_cu < 1 || blocks_per_cu > 32)
+    return fail(PYQED_EINVAL, "plan_set_class_tiling: need 0 <= cls < 4, target_elems >= 1, 1 <= blocks_per_cu <= 32");
+  PQA_HIP(hipDeviceSynchronize());  // the old queue may still be in use by queued launches
+  P->cls_target[cls] = target_elems;
+  P->cls_bpc[cls] = blocks_per_cu;
+  build_items(P);
+  return upload_items(P);
+}
+
+extern "C" int pyqed_heom_plan_get_class_tiling(const pyqed_heom_plan* P, int64_t* target_elems,
+                                               int32_t* blocks_per_cu) {
+  if (!P || !target_elems || !blocks_per_cu) return fail(PYQED_EINVAL, "plan_get_class_tiling: null argument");
+  for (int c = 0; c < kClasses; ++c) {
+    target_elems[c] = P->cls_target[c];
+    blocks_per_cu[c] = P->cls_bpc[c];
+  }
+  return PYQED_OK;
+}
+
 extern "C" int pyqed_heom_plan_set_variant(pyqed_heom_plan* P, int32_t variant) {
   if (!P || variant < 0 || variant > 3) return fail(PYQED_EINVAL, "plan_set_variant: variant must be in 0..3");
   P->variant = variant;
@@ -2934,8 +2969,8 @@ int launch(Kern k, int grid, size_t lds, hipStream_t s, const StageParams& p, in
   return PYQED_OK;
 }
 
-int stream_grid(const pyqed_heom_plan* P, int n_items) {
-  int g = std::min(n_items, P->num_cu * P->blocks_per_cu);
+int stream_grid(const pyqed_heom_plan* P, int n_items, int c) {
+  int g = std::min(n_items, P->num_cu * P->cls_bpc[c]);
   g = ((g + 7) / 8) * 8;
   return std::max(g, 8);
 }
@@ -2988,7 +3023,7 @@ int launch_stage_ti(pyqed_heom_plan* P, const StageParams& p0, hipStream_t st, i
     p.item_end = P->cls_begin[c + 1];
     p.use_xcd = 1;
     std::memcpy(p.xr, P->xr[c], sizeof(p.xr));
-    const int grid = stream_grid(P, n);
+    const int grid = stream_grid(P, n, c);
     int rc;
     // the projector kernel keeps neighbour rows (and columns) in LDS; when they do not fit, the
     // generic kernel (which any instance can use) takes the class

@@ -246,7 +246,21 @@ class HeomEngine:
     def tiling(self):
         t, b = C.c_int64(), C.c_int32()
         _lib.check(self.L.pyqed_heom_plan_get_tiling(self.plan, C.byref(t), C.byref(b)))
-        return {"target_elems": t.value, "blocks_per_cu": b.value, "variant": self.variant}
+        return {"target_elems": t.value, "blocks_per_cu": b.value, "variant": self.variant,
+                "classes": [list(x) for x in self.class_tiling()]}
+
+    def set_class_tiling(self, cls, target_elems, blocks_per_cu):
+        """Work-item size and blocks per CU of one kernel class (0 generic, 1 projector VALU, 2 MFMA 16 x 16,
+        3 MFMA 2 x 2); set_tiling sets every class."""
+        self._drop_graphs()
+        _lib.check(self.L.pyqed_heom_plan_set_class_tiling(self.plan, int(cls), int(target_elems), int(blocks_per_cu)))
+
+    def class_tiling(self):
+        """[(target_elems, blocks_per_cu)] per kernel class."""
+        t = np.zeros(NCLASSES, np.int64)
+        b = np.zeros(NCLASSES, np.int32)
+        _lib.check(self.L.pyqed_heom_plan_get_class_tiling(self.plan, t.ctypes.data, b.ctypes.data))
+        return [(int(x), int(y)) for x, y in zip(t, b)]
 
     def classes(self):
         """Streamed work items and full-matrix elements per kernel class (0 generic, 1 site projector with
@@ -300,6 +314,9 @@ class HeomEngine:
             hit = autotune.lookup(self, precision)
             if hit is not None:
                 self.set_tiling(*hit)
+                for c, tb in enumerate(autotune.lookup_classes(self, precision) or []):
+                    if tb:
+                        self.set_class_tiling(c, *tb)
                 self._tuned_hit = True
         ws = self.workspace(prec)
         _lib.check(self.L.pyqed_heom_run(

@@ -109,3 +109,36 @@ def test_advance_rejects_bad_field_and_traj():
     with pytest.raises(ValueError):
         eng.advance_graph(5, 0.01, "fp64", chunk=5)
     torch.cuda.synchronize()
+
+
+def test_class_tiling_changes_only_the_partition():
+    """Per-class work-item sizes and grids (the autotuner's per-class choice) change how the queue is
+    cut, not the arithmetic: results are bitwise equal to the plan-wide tiling's."""
+    import numpy as np
+    from pyqed_amd.heom.engine import HeomEngine
+    from pyqed_amd import workloads as W
+    from pyqed_amd.heom.bath import drude
+    insts = []
+    for p in W.batched_c3(24, seed=5):
+        e = drude(p["lam"], p["gam"], p["beta"], p["npsd"])
+        d = p["H"].shape[0]
+        bath = tuple(np.tile(a, d) for a in e) + (np.repeat(np.arange(d), len(e[3])),)
+        insts.append(dict(H=p["H"], Q=p["Q"], L=p["L"], rho0=p["rho0"], bath=bath))
+    runs = []
+    for per_class in (False, True):
+        eng = HeomEngine(insts, resident=False)
+        eng.autotune_cache = False
+        eng.set_tiling(4096, 8, 0)
+        items0 = eng.classes()["items"]
+        if per_class:
+            for c in range(4):
+                if items0[c]:
+                    eng.set_class_tiling(c, 1024 * (c + 1), 4 + 4 * (c % 2))
+            ct = eng.class_tiling()
+            assert all(ct[c] == (1024 * (c + 1), 4 + 4 * (c % 2)) for c in range(4) if items0[c])
+            items1 = eng.classes()["items"]
+            assert any(items1[c] != items0[c] for c in range(4) if items0[c])
+        runs.append(eng.run(0.01, 30, precision="mixed", save_every=10))
+        eng.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
