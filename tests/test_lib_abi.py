@@ -113,3 +113,37 @@ def test_rhs_combine_argument_checks_without_a_device():
     L = _lib.lib()
     assert L.pyqed_heom_rhs_combine(None, None, None, None, 0.0, None, 0.0, 0, None, 0, 0, None) == 1
     assert b"null plan or input" in L.pyqed_last_error()
+
+
+def test_lindblad_matrix_argument_checks_without_a_device():
+    """pyqed_lindblad_work_bytes sizes the per-instance workspace; pyqed_lindblad_run rejects bad
+    shapes, missing operators and a short workspace before any device work."""
+    from pyqed_amd import _lib
+    L = _lib.lib()
+    n = C.c_int64()
+    assert L.pyqed_lindblad_work_bytes(64, 2, 0, 10, C.byref(n)) == 0
+    assert n.value == 10 * (3 + 2) * 64 * 64 * 16
+    assert L.pyqed_lindblad_work_bytes(8, 1, 2, 3, C.byref(n)) == 0 and n.value == 3 * (3 + 1 + 2) * 64 * 16
+    assert L.pyqed_lindblad_work_bytes(0, 1, 0, 1, C.byref(n)) == 1
+    dummy = C.c_void_p(16)  # never dereferenced: every call below fails its checks first
+    args = dict(d=4, nc=1, Heff=dummy, HeffD=dummy, C=dummy, CD=dummy, ndrive=0, Hd=None, HdD=None, n_inst=2,
+                rho=dummy, field=None, nsteps=5, dt=0.1, n_obs=0, obs=None, obs_out=None, rho_out=None, save_every=1,
+                work=dummy, work_bytes=1 << 30, stream=None)
+
+    def call(**kw):
+        a = dict(args, **kw)
+        return L.pyqed_lindblad_run(*a.values())
+    assert call(d=0) == 1 and b"1 <= d" in L.pyqed_last_error()
+    assert call(C=None) == 1 and b"nc > 0 needs" in L.pyqed_last_error()
+    assert call(ndrive=1) == 1 and b"drives need" in L.pyqed_last_error()
+    assert call(n_obs=1) == 1 and b"bad observables" in L.pyqed_last_error()
+    assert call(nsteps=-1) == 1 and b"negative" in L.pyqed_last_error()
+    assert call(work_bytes=100) == 1 and b"workspace" in L.pyqed_last_error()
+
+
+def test_class_tiling_argument_checks_without_a_device():
+    from pyqed_amd import _lib
+    L = _lib.lib()
+    assert L.pyqed_heom_plan_set_class_tiling(None, 0, 1024, 8) == 1
+    assert b"plan_set_class_tiling" in L.pyqed_last_error()
+    assert L.pyqed_heom_plan_get_class_tiling(None, None, None) == 1
