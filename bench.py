@@ -15,6 +15,9 @@ Printed JSON line:
                  runs over the workload's full step count (2000 for the batch) on a seeded sample of
                  64 instances covering every (d, depth) stratum; after_timed_steps: the same over
                  every instance after the timed steps
+  error_vs_oracle  the same mixed trajectories against the CPU oracle (the reference's FP64 solver
+                 restated, oracle.heom.run) over the full step count, for the cheapest instance of as
+                 many (d, depth) strata as a 30 s CPU budget allows (with the CPU legs only)
   roofline       the dominant stage kernel (the kernel-class launch with the longest average time,
                  timed per launch with HIP events on its stream, in a pass of up to 10 steps after
                  the timed steps): algorithmic HBM bytes per launch /
@@ -1035,12 +1038,72 @@ def full_length_error(insts, dt, nt, torch, n=64, save_every=20):
     pops, cohs = zip(*[max_errs(a, b) for a, b in zip(traj["mixed"], traj["fp64"])])
     strata = {(p["H"].shape[0], p["L"]) for p in sub}
     worst = int(np.argmax(pops))
+    out = {"population": max(pops), "coherence": max(cohs), "steps": nt, "save_every": save_every,
+           "instances": len(sub), "strata": len(strata), "d_max": max(d for d, _ in strata),
+           "worst_instance": {"index": idx[worst], "d": sub[worst]["H"].shape[0], "depth": sub[worst]["L"],
+                              "population": pops[worst]},
+           "seconds": t_leg, "reference": "the same kernels in plain FP64 (packed state, FP64 stage inputs and "
+                                          "RK4 register), which the golden tests pin to the reference at 1e-12"}
+    return out, sub, traj["mixed"]
+
+
+def oracle_error(sub, mixed, dt, nt, save_every=20, budget_s=30.0):
+    """The mixed GPU trajectories against the CPU oracle (oracle.heom.run: the reference's RK4 and
+    right-hand side restated in numpy FP64, bit-identical to the reference on the spin-boson
+    trajectory) over the full nt steps: the cheapest instance of each (d, depth) stratum first, as many
+    strata as the CPU budget allows (an instance that would not finish within it is dropped)."""
+    from math import comb
+    from oracle import heom as O
+    cost = [comb(len(p["bath"][3]) + p["L"], p["L"]) * p["H"].shape[0] ** 2 * len(p["bath"][3]) for p in sub]
+    order = sorted(range(len(sub)), key=lambda k: cost[k])
+    seen, pops, cohs, used = set(), [], [], []
+    rate = None
+    t_start = time.perf_counter()
+    for k in order:
+        p = sub[k]
+        key = (p["H"].shape[0], p["L"])
+        if key in seen:
+            continue
+        left = budget_s - (time.perf_counter() - t_start)
+        if left <= 0 or (rate is not None and cost[k] * nt * rate > 2 * left):
+            continue
+        etal, etar, etaa, expn, mode = p["bath"]
+        Q = [np.asarray(q, np.complex128) for q in p["Q"]]
+        t0 = time.perf_counter()
+        if all(np.count_nonzero(q - np.diag(np.diag(q))) == 0 for q in Q):
+            # diagonal couplings: the oracle's row/column-scaling form of the same right-hand side
+            hier = O.Hierarchy(len(expn), p["L"])
+            terms = O.DiagTerms(hier, (etal, etar, etaa, expn), mode, np.array([np.diag(q) for q in Q]))
+            H = np.asarray(p["H"], np.complex128)
+            y = np.zeros((hier.nmax,) + H.shape, np.complex128)
+            y[0] = p["rho0"]
+            ref = [y[0].copy()]
+            for i in range(nt):
+                y = O.rk4_step(y, lambda x, t: O.rhs_vec_diag(x, H, terms), dt, i * dt)
+                if (i + 1) % save_every == 0:
+                    ref.append(y[0].copy())
+                if i % 100 == 99 and time.perf_counter() - t_start > budget_s:
+                    break  # over budget: this instance is dropped
+            if len(ref) < nt // save_every + 1:
+                break
+            ref = np.array(ref)
+        else:
+            _, ref = O.run(p["H"], Q, (etal, etar, etaa, expn), mode, p["L"], p["rho0"], dt, nt,
+                           save_every=save_every)
+        el = time.perf_counter() - t0
+        rate = el / (cost[k] * nt)  # per unit of cost, from the latest (largest) instance
+        pop, coh = max_errs(mixed[k], ref)
+        pops.append(pop)
+        cohs.append(coh)
+        used.append(key)
+        seen.add(key)
+    if not used:
+        return None
     return {"population": max(pops), "coherence": max(cohs), "steps": nt, "save_every": save_every,
-            "instances": len(sub), "strata": len(strata), "d_max": max(d for d, _ in strata),
-            "worst_instance": {"index": idx[worst], "d": sub[worst]["H"].shape[0], "depth": sub[worst]["L"],
-                               "population": pops[worst]},
-            "seconds": t_leg, "reference": "the same kernels in plain FP64 (packed state, FP64 stage inputs and "
-                                           "RK4 register), which the golden tests pin to the reference at 1e-12"}
+            "instances": len(used), "strata": len(used), "d_max": max(d for d, _ in used),
+            "depth_max": max(L for _, L in used), "seconds": time.perf_counter() - t_start,
+            "oracle": "the numpy FP64 restatement of deom.py:641-766 (oracle.heom; its row/column-scaling form for "
+                      "diagonal couplings)"}
 
 
 def fp64_batched(insts, dt, warmup, steps, torch):
@@ -1210,8 +1273,11 @@ def bench_heom(args, torch):
         if not args.no_full_error and args.workload in FULL_STEPS:
             nt = FULL_STEPS[args.workload]
             print(f"[bench] full-length accuracy leg ({nt} steps)", file=sys.stderr, flush=True)
-            out["error_vs_fp64"] = full_length_error(insts, dt, nt, torch)
+            out["error_vs_fp64"], sample, mixed_traj = full_length_error(insts, dt, nt, torch)
             out["error_vs_fp64"]["after_timed_steps"] = short
+            if not args.no_cpu:
+                print("[bench] oracle accuracy leg", file=sys.stderr, flush=True)
+                out["error_vs_oracle"] = oracle_error(sample, mixed_traj, dt, nt)
         else:
             out["error_vs_fp64"] = short
     if not args.no_cpu:
