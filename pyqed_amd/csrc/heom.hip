@@ -87,7 +87,9 @@ struct WorkItem {
 // F_SITESORT: site(k) nondecreasing; F_TINY: at most kThreads elements, d <= kTinyD, diagonal couplings and
 // at most kTinyEnt neighbour entries per ADO (the resident kernel keeps one element per thread in registers);
 // F_TINY4: at most 4 neighbour entries per ADO
-enum { F_SDIP = 1, F_CDIP = 2, F_DENSE = 4, F_HERM = 8, F_SITESORT = 16, F_TINY = 32, F_TINY4 = 64 };
+// F_REALH: H and the system dipole are real, so H(t) is real and the MFMA commutator skips the products
+// with Im H (exact zeros)
+enum { F_SDIP = 1, F_CDIP = 2, F_DENSE = 4, F_HERM = 8, F_SITESORT = 16, F_TINY = 32, F_TINY4 = 64, F_REALH = 128 };
 constexpr int kTinyD = 8;
 constexpr int kTinyEnt = 8;
 
@@ -1012,40 +1014,50 @@ void heom_proj_kernel(StageParams p) {
         const cpx<T>* sHa = sH + hz;
         // 2 x 2 tiles: one k-chunk at a time keeps 16 operands and 32 accumulators live
         constexpr int kUnrollKC = NT == 1 ? 4 : 1;
+        // 2 x 2 tiles: k-chunks past d hold only zero padding and are skipped (6 of 8 at d = 24)
+        const int nkc = NT == 1 ? 4 : (d + 3) >> 2;
+        // real H (F_REALH): the four products with Im H are exact zeros and are skipped
+        auto kloop = [&](auto RH) {
+          constexpr bool kRealH = decltype(RH)::value;
 #pragma unroll kUnrollKC
-        for (int kc = 0; kc < 4 * NT; ++kc) {
-          const int kk = 4 * kc + gk;
-          float xar[NT], xai[NT], har[NT], hai[NT], xbr[NT], xbi[NT], hbr[NT], hbi[NT];
+          for (int kc = 0; kc < nkc; ++kc) {
+            const int kk = 4 * kc + gk;
+            float xar[NT], xai[NT], har[NT], hai[NT], xbr[NT], xbi[NT], hbr[NT], hbi[NT];
 #pragma unroll
-          for (int t = 0; t < NT; ++t) {
-            const int g = 16 * t + gi;
-            const bool ok = g < d && kk < d;
-            const cpx<T> xb = ok ? w.X[kk * rs + g] : mk<T>(0, 0);  // B-layout X[kk][g]
-            const cpx<T> xa = ok ? w.X[g * rs + kk] : mk<T>(0, 0);  // A-layout X[g][kk]
-            const cpx<T> ha = ok ? sHa[g * rs + kk] : mk<T>(0, 0);  // A-layout H[g][kk]
-            const cpx<T> hb = ok ? sHa[kk * rs + g] : mk<T>(0, 0);  // B-layout H[kk][g]
-            xbr[t] = (float)xb.x, xbi[t] = (float)xb.y, xar[t] = (float)xa.x, xai[t] = (float)xa.y;
-            har[t] = (float)ha.x, hai[t] = (float)ha.y, hbr[t] = (float)hb.x, hbi[t] = (float)hb.y;
-          }
-#pragma unroll
-          for (int ti = 0; ti < NT; ++ti)
-#pragma unroll
-            for (int tj = 0; tj < NT; ++tj) {
-              floatx4 r = aR[ti][tj], m = aI[ti][tj];
-              // + H X: rows of tile ti from H, columns of tile tj from X
-              r = __builtin_amdgcn_mfma_f32_16x16x4f32(har[ti], xbr[tj], r, 0, 0, 0);
-              r = __builtin_amdgcn_mfma_f32_16x16x4f32(-hai[ti], xbi[tj], r, 0, 0, 0);
-              m = __builtin_amdgcn_mfma_f32_16x16x4f32(har[ti], xbi[tj], m, 0, 0, 0);
-              m = __builtin_amdgcn_mfma_f32_16x16x4f32(hai[ti], xbr[tj], m, 0, 0, 0);
-              // - X H
-              r = __builtin_amdgcn_mfma_f32_16x16x4f32(xar[ti], -hbr[tj], r, 0, 0, 0);
-              r = __builtin_amdgcn_mfma_f32_16x16x4f32(xai[ti], hbi[tj], r, 0, 0, 0);
-              m = __builtin_amdgcn_mfma_f32_16x16x4f32(xar[ti], -hbi[tj], m, 0, 0, 0);
-              m = __builtin_amdgcn_mfma_f32_16x16x4f32(xai[ti], -hbr[tj], m, 0, 0, 0);
-              aR[ti][tj] = r;
-              aI[ti][tj] = m;
+            for (int t = 0; t < NT; ++t) {
+              const int g = 16 * t + gi;
+              const bool ok = g < d && kk < d;
+              const cpx<T> xb = ok ? w.X[kk * rs + g] : mk<T>(0, 0);  // B-layout X[kk][g]
+              const cpx<T> xa = ok ? w.X[g * rs + kk] : mk<T>(0, 0);  // A-layout X[g][kk]
+              const cpx<T> ha = ok ? sHa[g * rs + kk] : mk<T>(0, 0);  // A-layout H[g][kk]
+              const cpx<T> hb = ok ? sHa[kk * rs + g] : mk<T>(0, 0);  // B-layout H[kk][g]
+              xbr[t] = (float)xb.x, xbi[t] = (float)xb.y, xar[t] = (float)xa.x, xai[t] = (float)xa.y;
+              har[t] = (float)ha.x, hai[t] = (float)ha.y, hbr[t] = (float)hb.x, hbi[t] = (float)hb.y;
             }
-        }
+#pragma unroll
+            for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+              for (int tj = 0; tj < NT; ++tj) {
+                floatx4 r = aR[ti][tj], m = aI[ti][tj];
+                // + H X: rows of tile ti from H, columns of tile tj from X
+                r = __builtin_amdgcn_mfma_f32_16x16x4f32(har[ti], xbr[tj], r, 0, 0, 0);
+                if constexpr (!kRealH) r = __builtin_amdgcn_mfma_f32_16x16x4f32(-hai[ti], xbi[tj], r, 0, 0, 0);
+                m = __builtin_amdgcn_mfma_f32_16x16x4f32(har[ti], xbi[tj], m, 0, 0, 0);
+                if constexpr (!kRealH) m = __builtin_amdgcn_mfma_f32_16x16x4f32(hai[ti], xbr[tj], m, 0, 0, 0);
+                // - X H
+                r = __builtin_amdgcn_mfma_f32_16x16x4f32(xar[ti], -hbr[tj], r, 0, 0, 0);
+                if constexpr (!kRealH) r = __builtin_amdgcn_mfma_f32_16x16x4f32(xai[ti], hbi[tj], r, 0, 0, 0);
+                if constexpr (!kRealH) m = __builtin_amdgcn_mfma_f32_16x16x4f32(xar[ti], -hbi[tj], m, 0, 0, 0);
+                m = __builtin_amdgcn_mfma_f32_16x16x4f32(xai[ti], -hbr[tj], m, 0, 0, 0);
+                aR[ti][tj] = r;
+                aI[ti][tj] = m;
+              }
+          }
+        };
+        if (I.flags & F_REALH)
+          kloop(std::true_type());
+        else
+          kloop(std::false_type());
 #pragma unroll
         for (int ti = 0; ti < NT; ++ti)
 #pragma unroll
@@ -1445,8 +1457,9 @@ __global__ __launch_bounds__(kThreads, PQA_WIDE_BLOCK_WAVES) void heom_wide_kern
       {
         floatx4 aR = {0.f, 0.f, 0.f, 0.f}, aI = {0.f, 0.f, 0.f, 0.f};
         const int ga = 16 * ti + gi, gb = 16 * tj + gi;
+        const int nkc = (d + 3) >> 2;  // k-chunks past d hold only zero padding
 #pragma unroll 1
-        for (int kc = 0; kc < 8; ++kc) {
+        for (int kc = 0; kc < nkc; ++kc) {
           const int kk = 4 * kc + gk;
           const bool oka = ga < d && kk < d, okb = gb < d && kk < d;
           const cpx<T> xa = oka ? X[ga * rs + kk] : mk<T>(0, 0);   // A-layout X[ga][kk]
@@ -2678,6 +2691,9 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
       return fail(PYQED_EINVAL, "coefficient pools exceed 2^31 entries");
     }
     {
+      bool realh = true;
+      for (int e = 0; e < I.d2 && realh; ++e) realh = s.H[2 * e + 1] == 0.0 && (!s.sdip || s.sdip[2 * e + 1] == 0.0);
+      if (realh) I.flags |= F_REALH;
       int maxrow = 0;
       for (int a = 0; a < t.nmax; ++a) maxrow = std::max(maxrow, t.rowptr[a + 1] - t.rowptr[a]);
       if (!dense && s.d <= kTinyD && (int64_t)t.nmax * I.d2 <= kThreads && maxrow <= kTinyEnt) {
