@@ -639,6 +639,11 @@ constexpr int kChunk = PQA_CHUNK;  // neighbour loads per lane issued together
 // the neighbour-row requests (both round trips overlap the commutator)
 #define PQA_RK4_HOIST 1
 #endif
+#ifndef PQA_HALF_COMMUTATOR
+// packed Hermitian MFMA forms: 1 form only P = H X on MFMA and complete the commutator from the
+// output tile; 0 form H X - X H on MFMA
+#define PQA_HALF_COMMUTATOR 1
+#endif
 #ifndef PQA_ROWS_DMA
 // wide projector class: neighbour rows by LDS DMA (global_load_lds_dword) instead of register chunks
 #define PQA_ROWS_DMA 1
@@ -808,6 +813,9 @@ void heom_proj_kernel(StageParams p) {
   // Hermitian DMA plans keep, per entry, the row's index in the instance (neighbour * d + site) in
   // place of the site.
   constexpr bool kDMA = PQA_ROWS_DMA && NT == 2 && std::is_same<TI, T>::value;
+  // packed Hermitian MFMA forms: with H and the stage input X Hermitian, X H = (H X)^H, so only
+  // P = H X is formed on MFMA and the commutator is completed from the output tile
+  constexpr bool kHalf = PQA_HALF_COMMUTATOR && MFMA && PACK && HERM;
   int it0, itstep, itend;
   item_range(p, it0, itend, itstep);
   for (int it = it0; it < itend; it += itstep) {
@@ -1016,7 +1024,8 @@ void heom_proj_kernel(StageParams p) {
         constexpr int kUnrollKC = NT == 1 ? 4 : 1;
         // 2 x 2 tiles: k-chunks past d hold only zero padding and are skipped (6 of 8 at d = 24)
         const int nkc = NT == 1 ? 4 : (d + 3) >> 2;
-        // real H (F_REALH): the four products with Im H are exact zeros and are skipped
+        // real H (F_REALH): the four products with Im H are exact zeros and are skipped.  Packed
+        // Hermitian plans form only P = H X (kHalf)
         auto kloop = [&](auto RH) {
           constexpr bool kRealH = decltype(RH)::value;
 #pragma unroll kUnrollKC
@@ -1031,7 +1040,11 @@ void heom_proj_kernel(StageParams p) {
               const cpx<T> xa = ok ? w.X[g * rs + kk] : mk<T>(0, 0);  // A-layout X[g][kk]
               const cpx<T> ha = ok ? sHa[g * rs + kk] : mk<T>(0, 0);  // A-layout H[g][kk]
               const cpx<T> hb = ok ? sHa[kk * rs + g] : mk<T>(0, 0);  // B-layout H[kk][g]
-              xbr[t] = (float)xb.x, xbi[t] = (float)xb.y, xar[t] = (float)xa.x, xai[t] = (float)xa.y;
+              // kHalf: P is formed from the Hermitian part of X, whose diagonal is real.  The stored
+              // diagonal carries rounding-level imaginary parts, which H X - X H cancels exactly and
+              // H X alone would turn into a trace drift of 2 Im(H_ii X_ii)
+              xbr[t] = (float)xb.x, xbi[t] = (kHalf && kk == g) ? 0.f : (float)xb.y;
+              xar[t] = (float)xa.x, xai[t] = (float)xa.y;
               har[t] = (float)ha.x, hai[t] = (float)ha.y, hbr[t] = (float)hb.x, hbi[t] = (float)hb.y;
             }
 #pragma unroll
@@ -1045,10 +1058,12 @@ void heom_proj_kernel(StageParams p) {
                 m = __builtin_amdgcn_mfma_f32_16x16x4f32(har[ti], xbi[tj], m, 0, 0, 0);
                 if constexpr (!kRealH) m = __builtin_amdgcn_mfma_f32_16x16x4f32(hai[ti], xbr[tj], m, 0, 0, 0);
                 // - X H
-                r = __builtin_amdgcn_mfma_f32_16x16x4f32(xar[ti], -hbr[tj], r, 0, 0, 0);
-                if constexpr (!kRealH) r = __builtin_amdgcn_mfma_f32_16x16x4f32(xai[ti], hbi[tj], r, 0, 0, 0);
-                if constexpr (!kRealH) m = __builtin_amdgcn_mfma_f32_16x16x4f32(xar[ti], -hbi[tj], m, 0, 0, 0);
-                m = __builtin_amdgcn_mfma_f32_16x16x4f32(xai[ti], -hbr[tj], m, 0, 0, 0);
+                if constexpr (!kHalf) {
+                  r = __builtin_amdgcn_mfma_f32_16x16x4f32(xar[ti], -hbr[tj], r, 0, 0, 0);
+                  if constexpr (!kRealH) r = __builtin_amdgcn_mfma_f32_16x16x4f32(xai[ti], hbi[tj], r, 0, 0, 0);
+                  if constexpr (!kRealH) m = __builtin_amdgcn_mfma_f32_16x16x4f32(xar[ti], -hbi[tj], m, 0, 0, 0);
+                  m = __builtin_amdgcn_mfma_f32_16x16x4f32(xai[ti], -hbr[tj], m, 0, 0, 0);
+                }
                 aR[ti][tj] = r;
                 aI[ti][tj] = m;
               }
@@ -1067,7 +1082,9 @@ void heom_proj_kernel(StageParams p) {
               const int i = 16 * ti + 4 * gk + r, j = 16 * tj + gi;
               cpx<T>& a2 = acc[(ti * NT + tj) * 4 + r];
               a2 = mul_mi(mk<T>((T)aR[ti][tj][r], (T)aI[ti][tj][r]));
-              if (i < d && j < d) cmsub(a2, damp, w.X[i * rs + j]);
+              // kHalf: -i P - (damp / 2) X here; the reader adds the conjugate of the (j, i) entry
+              // (damp is real in Hermitian plans and X Hermitian, so the two halves sum to damp X)
+              if (i < d && j < d) cmsub(a2, kHalf ? mk<T>(damp.x * (T)0.5, damp.y * (T)0.5) : damp, w.X[i * rs + j]);
             }
       }
       if constexpr (kDMA) {
@@ -1144,6 +1161,11 @@ void heom_proj_kernel(StageParams p) {
             if (pk < np) {
               const int ij = sIJ[pk], i = ij >> 8, j = ij & 255;
               au[q] = w.Tt[i * rs + j];
+              if constexpr (kHalf) {  // -i (P - P^H) - damp X = T(i, j) + conj(T(j, i))
+                const cpx<T> tt = w.Tt[j * rs + i];
+                au[q].x += tt.x;
+                au[q].y -= tt.y;
+              }
               neighbours(au[q], i, j);
             }
           }
