@@ -601,7 +601,7 @@ __global__ __launch_bounds__(kThreads) void heom_stage_kernel(StageParams p) {
 // The neighbour list is sorted by site in LDS (ballot counting sort), so element (i, j) reads only
 // its two groups from the row buffer: sum_{site(t)=i} cL_t R_t(j) + sum_{site(t)=j} cR_t R_t(i)^*
 // (Hermitian ADOs: X'(i, s) = conj(X'(s, i))).  The commutator -i[H, X] runs on FP32 MFMA 16x16x4
-// for 12 <= d <= 16 and on VALU otherwise, while the round-trip-2 loads are in flight.
+// for PQA_MFMA_DMIN <= d <= 16 and on VALU otherwise, while the round-trip-2 loads are in flight.
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxEnt = 64;  // neighbour entries per ADO handled by the projector path
@@ -647,6 +647,10 @@ constexpr int kChunk = PQA_CHUNK;  // neighbour loads per lane issued together
 #ifndef PQA_ROWS_DMA
 // wide projector class: neighbour rows by LDS DMA (global_load_lds_dword) instead of register chunks
 #define PQA_ROWS_DMA 1
+#endif
+#ifndef PQA_ROWS_DMA_NARROW
+// the same for the packed 16 x 16 MFMA class (0: register chunks)
+#define PQA_ROWS_DMA_NARROW 0
 #endif
 #ifndef PQA_PACK_HOIST
 // packed MFMA forms: 0 request the RK4 operands of an ADO's upper elements once its neighbour rows
@@ -812,7 +816,7 @@ void heom_proj_kernel(StageParams p) {
   // shortens the 24-site chain's step by 6%; the 16 x 16 MFMA and VALU classes ran 1-10% slower with it.
   // Hermitian DMA plans keep, per entry, the row's index in the instance (neighbour * d + site) in
   // place of the site.
-  constexpr bool kDMA = PQA_ROWS_DMA && NT == 2 && std::is_same<TI, T>::value;
+  constexpr bool kDMA = PQA_ROWS_DMA && (NT == 2 || (PQA_ROWS_DMA_NARROW && MFMA && PACK)) && std::is_same<TI, T>::value;
   // packed Hermitian MFMA forms: with H and the stage input X Hermitian, X H = (H X)^H, so only
   // P = H X is formed on MFMA and the commutator is completed from the output tile
   constexpr bool kHalf = PQA_HALF_COMMUTATOR && MFMA && PACK && HERM;
@@ -928,8 +932,9 @@ void heom_proj_kernel(StageParams p) {
         const int J = total >> 6;
         const float* src = (const float*)xi;
         if constexpr (HERM) {
-          // rows only: lane position (t, k) in the buffer advanced by 64 dwords per group (m >= 36, so at
-          // most two wraps), the row start read once per group from the per-entry row index
+          // rows only: lane position (t, k) in the buffer advanced by 64 dwords per group (wide class:
+          // m >= 36, so at most two wraps; 16 x 16 class: m >= 18, up to four), the row start read once
+          // per group from the per-entry row index
           float* dst = (float*)w.R;
           int t = lane / m, k = lane - (lane / m) * m;
           const int rw = kW * d;  // dwords of a row
@@ -938,13 +943,20 @@ void heom_proj_kernel(StageParams p) {
             const int64_t off = k < rw ? (int64_t)w.st[t] * rw + k : 0;
             __builtin_amdgcn_global_load_lds(src + off, (__attribute__((address_space(3))) void*)(dst + 64 * j), 4, 0, 0);
             k += 64;
-            if (k >= m) {
-              k -= m;
-              ++t;
-            }
-            if (k >= m) {
-              k -= m;
-              ++t;
+            if constexpr (NT == 2) {
+              if (k >= m) {
+                k -= m;
+                ++t;
+              }
+              if (k >= m) {
+                k -= m;
+                ++t;
+              }
+            } else {
+              while (k >= m) {  // at most 64 / m + 1 <= 4 times (m = 2 (d + 1) >= 18)
+                k -= m;
+                ++t;
+              }
             }
           }
           if (64 * J + lane < total) tail[0] = src[k < rw ? (int64_t)w.st[t] * rw + k : 0];
@@ -1581,7 +1593,7 @@ __global__ __launch_bounds__(kThreads, PQA_WIDE_BLOCK_WAVES) void heom_wide_kern
 }
 
 // ------------------------------------------------------------------ pipelined site-projector kernel
-// The dominant class (12 <= d <= 16, Hermitian hierarchy, FP32 right-hand side) in a two-deep
+// The dominant class (PQA_MFMA_DMIN <= d <= 16, Hermitian hierarchy, FP32 right-hand side) in a two-deep
 // software pipeline: while ADO a is gathered and written, the neighbour rows of ADO a+1 are in
 // flight and the neighbour list and matrix of ADO a+2 are being fetched.  Per-wave LDS buffers
 // (own matrix, neighbour list, row buffer) are doubled.  Loads are issued in the order
