@@ -148,6 +148,13 @@ struct StageParams {
   double2* cacc;
   double calpha, cbeta;
   int32_t cinit;
+  // projector classes with a single streamed instance (one_per > 0): its record is inst0 and item k of
+  // the class holds ADOs [k one_per, min((k + 1) one_per, nmax)), so neither table is read; its
+  // neighbour lists are read from ent_fix at fix_stride entries per ADO (the row pointers give only
+  // their lengths).  Each saves a dependent round trip per work item.
+  InstDev inst0;
+  int32_t one_per, fix_stride;
+  const int2* ent_fix;
 };
 
 // LDS carve-up shared by both kernels.
@@ -542,6 +549,16 @@ __device__ __forceinline__ void item_range(const StageParams& p, int& it0, int& 
   }
 }
 
+// work item `it` of a single-instance projector class (StageParams::one_per)
+__device__ __forceinline__ WorkItem one_item(const StageParams& p, int it) {
+  WorkItem w;
+  w.inst = 0;
+  w.ado0 = (it - p.item_begin) * p.one_per;
+  w.nado = min(p.one_per, p.inst0.nmax - w.ado0);
+  w.pad = 0;
+  return w;
+}
+
 // ------------------------------------------------------------------ streamed stage kernel
 // STAGE 0..3: RK4 stage; STAGE 4: out = RHS(xin) only (FP64 operator check).
 template <typename T, typename TI, typename TO, int STAGE, bool PACK>
@@ -774,7 +791,9 @@ constexpr int proj_waves(bool mfma, bool f32, bool pack, int stage, int nt) {
 }
 
 // NT: MFMA tiles per dimension, 1 for d <= 16 (one 16 x 16 tile) or 2 for d <= 32 (2 x 2 tiles of 16).
-template <typename T, typename TI, typename TO, int STAGE, bool MFMA, bool HERM, bool PACK, int NT>
+// ONE: the class holds a single instance (StageParams::one_per, inst0, ent_fix); a compile-time form so
+// that the table-driven forms keep their registers.
+template <typename T, typename TI, typename TO, int STAGE, bool MFMA, bool HERM, bool PACK, int NT, bool ONE = false>
 __global__ __launch_bounds__(kThreads, proj_waves(MFMA, std::is_same<T, float>::value, PACK, STAGE, NT))
 void heom_proj_kernel(StageParams p) {
   constexpr bool kDelta = std::is_same<TO, float>::value;  // mixed precision: FP32 RK4 increment
@@ -823,8 +842,8 @@ void heom_proj_kernel(StageParams p) {
   int it0, itstep, itend;
   item_range(p, it0, itend, itstep);
   for (int it = it0; it < itend; it += itstep) {
-    const WorkItem wi = ro_load(p.items + it);
-    const InstDev I = ro_load(p.insts + wi.inst);
+    const WorkItem wi = ONE ? one_item(p, it) : ro_load(p.items + it);
+    const InstDev I = ONE ? p.inst0 : ro_load(p.insts + wi.inst);
     const int d = I.d, d2 = I.d2;
     // floor(idx / d) = (idx * dmag) >> 20 for idx < 2^11, d <= 32 (dmag*d - 2^20 < d < 2^20 / 2^11)
     const uint32_t dmag = ((1u << 20) + d - 1) / d;
@@ -840,10 +859,16 @@ void heom_proj_kernel(StageParams p) {
     int2 en_n = make_int2(0, 0);
     cpx_store<TI> xt_n[kPF];
     auto prefetch = [&](int an) {
-      const int r0 = ro_int(p.rowptr, I.rowptr_off + an);
-      nent_n = ro_int(p.rowptr, I.rowptr_off + an + 1) - r0;
-      en_n = make_int2(0, 0);
-      if (lane < nent_n) en_n = p.ent[I.ent_off + r0 + lane];
+      if constexpr (ONE) {
+        // the list's address does not depend on the row pointers, which only give its length
+        nent_n = ro_int(p.rowptr, I.rowptr_off + an + 1) - ro_int(p.rowptr, I.rowptr_off + an);
+        en_n = lane < p.fix_stride ? p.ent_fix[(int64_t)an * p.fix_stride + lane] : make_int2(0, 0);
+      } else {
+        const int r0 = ro_int(p.rowptr, I.rowptr_off + an);
+        nent_n = ro_int(p.rowptr, I.rowptr_off + an + 1) - r0;
+        en_n = make_int2(0, 0);
+        if (lane < nent_n) en_n = p.ent[I.ent_off + r0 + lane];
+      }
       // own matrix: its first 64 kPF elements (all of it for d <= 16 NT)
 #pragma unroll
       for (int q = 0; q < kPF; ++q) {
@@ -2289,6 +2314,11 @@ struct pyqed_heom_plan {
   WorkItem* d_items = nullptr;
   int32_t* d_rowptr = nullptr;
   int2* d_ent = nullptr;
+  // projector classes with a single streamed instance: that instance, and its neighbour lists at a fixed
+  // stride (StageParams::one_per, ent_fix)
+  int one_inst[kClasses] = {-1, -1, -1, -1};
+  int fix_stride[kClasses] = {0, 0, 0, 0};
+  int2* d_entfix[kClasses] = {nullptr, nullptr, nullptr, nullptr};
   double2* d_coef = nullptr;
   int32_t* d_mode = nullptr;
   double2* d_mat = nullptr;
@@ -2411,6 +2441,7 @@ void destroy_plan(pyqed_heom_plan* P) {
   (void)hipFree(P->d_items);
   (void)hipFree(P->d_rowptr);
   (void)hipFree(P->d_ent);
+  for (int c = 0; c < kClasses; ++c) (void)hipFree(P->d_entfix[c]);
   (void)hipFree(P->d_coef);
   (void)hipFree(P->d_mode);
   (void)hipFree(P->d_mat);
@@ -2784,9 +2815,36 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     destroy_plan(P);
     return fail(PYQED_EUNSUPPORTED, "instance too large for the streamed kernel's LDS budget");
   }
+  // ---- projector classes with a single streamed instance: neighbour lists at a fixed stride
+  std::vector<std::vector<int2>> entfix(kClasses);
+  for (int c = 1; c < kClasses; ++c) {
+    int only = -1, count = 0;
+    for (int i = 0; i < n_inst; ++i)
+      if (!is_res[i] && P->inst[i].path == c) only = i, ++count;
+    if (count != 1) continue;
+    const InstDev& I = P->inst[only];
+    const int S = I.K + std::min(I.K, I.L);  // entries per ADO at most (K plus, min(K, L) minus)
+    if ((int64_t)I.nmax * S * (int64_t)sizeof(int2) > (int64_t(1) << 28)) continue;
+    std::vector<int2>& f = entfix[c];
+    f.assign((size_t)I.nmax * S, make_int2(-1, 0));
+    bool fits = true;
+    for (int a = 0; a < I.nmax && fits; ++a) {
+      const int r0 = rowptr[I.rowptr_off + a], r1 = rowptr[I.rowptr_off + a + 1];
+      fits = r1 - r0 <= S;
+      for (int r = r0; r < r1 && fits; ++r) f[(size_t)a * S + (r - r0)] = ent[I.ent_off + r];
+    }
+    if (!fits) {
+      f.clear();
+      continue;
+    }
+    P->one_inst[c] = only;
+    P->fix_stride[c] = S;
+  }
   // ---- upload
   int rc = 0;
-  if ((rc = upload(P->inst, &P->d_inst)) || (rc = upload_items(P)) ||
+  for (int c = 1; c < kClasses && rc == 0; ++c)
+    if (!entfix[c].empty()) rc = upload(entfix[c], &P->d_entfix[c]);
+  if (rc || (rc = upload(P->inst, &P->d_inst)) || (rc = upload_items(P)) ||
       (rc = upload(rowptr, &P->d_rowptr)) || (rc = upload(ent, &P->d_ent)) || (rc = upload(coef, &P->d_coef)) ||
       (rc = upload(mode, &P->d_mode)) || (rc = upload(mat, &P->d_mat)) || (rc = upload(P->resident, &P->d_res)) ||
       (rc = upload(sites, &P->d_site)) ||
@@ -3062,12 +3120,20 @@ int launch_stage_ti(pyqed_heom_plan* P, const StageParams& p0, hipStream_t st, i
     const int n = P->cls_begin[c + 1] - P->cls_begin[c];
     if (n == 0) continue;
     StageParams p = p0;
+    bool one = false;  // a single streamed instance in this projector class
     if (c > 0) {  // projector kernels: LDS dimensions of this class
       p.d2pad = P->pd[c].d2pad;
       p.rowbuf = P->pd[c].rowbuf;
       p.colbuf = P->pd[c].colbuf;
       p.Kmax = P->pd[c].Kmax;
       p.Lmax = P->pd[c].Lmax;
+      if (P->one_inst[c] >= 0) {  // the class's items are consecutive ADO ranges of one instance
+        one = true;
+        p.inst0 = P->inst[P->one_inst[c]];
+        p.one_per = P->items[P->cls_begin[c]].nado;
+        p.ent_fix = P->d_entfix[c];
+        p.fix_stride = P->fix_stride[c];
+      }
     }
     p.item_begin = P->cls_begin[c];
     p.item_end = P->cls_begin[c + 1];
@@ -3090,8 +3156,12 @@ int launch_stage_ti(pyqed_heom_plan* P, const StageParams& p0, hipStream_t st, i
         rc = launch(heom_proj_kernel<T, TI, T, S, F32, true, true, 1>, grid, plds, st, p);
       else if (c == 3 && F32 && (P->variant & 2) && wlds <= (size_t)P->max_lds)
         rc = launch(heom_wide_kernel<S>, grid, wlds, st, p);
+      else if (c == 3 && F32 && one)
+        rc = launch(heom_proj_kernel<T, TI, T, S, F32, true, true, F32 ? 2 : 1, F32>, grid, plds, st, p);
       else if (c == 3 && F32)
         rc = launch(heom_proj_kernel<T, TI, T, S, F32, true, true, F32 ? 2 : 1>, grid, plds, st, p);
+      else if (F32 && one)
+        rc = launch(heom_proj_kernel<T, TI, T, S, false, true, true, 1, F32>, grid, plds, st, p);
       else
         rc = launch(heom_proj_kernel<T, TI, T, S, false, true, true, 1>, grid, plds, st, p);
     } else if (P->hermitian && c == 2 && F32 && (P->variant & 1) && pipe_lds_bytes(P, c) <= (size_t)P->max_lds) {
