@@ -267,7 +267,9 @@ def kernel_name(stage, cls, precision, eng):
     mfma = "true" if (cls >= 2 and precision == "mixed") else "false"
     herm = "true" if eng.hermitian else "false"
     nt = 2 if (cls == 3 and precision == "mixed") else 1
-    return f"heom_proj_kernel<{T}, {TI}, {T}, {stage}, {mfma}, {herm}, {pack}, {nt}>"
+    # single-instance form: FP32 packed classes 1 and 3 when the class's streamed work is one instance
+    one = precision == "mixed" and eng.packed and cls in (1, 3) and eng.class_single()[cls] >= 0
+    return f"heom_proj_kernel<{T}, {TI}, {T}, {stage}, {mfma}, {herm}, {pack}, {nt}, {'true' if one else 'false'}>"
 
 
 def measured_traffic(name, key, workload, packed, tiling):

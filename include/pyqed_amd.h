@@ -192,6 +192,11 @@ int pyqed_heom_plan_set_variant(pyqed_heom_plan* plan, int32_t variant);
  * 3 the same on 2 x 2 tiles (17 <= d <= 32)): items and elements
  * (host arrays [4], either may be NULL). */
 int pyqed_heom_plan_classes(const pyqed_heom_plan* plan, int64_t* items, int64_t* elems);
+/* Per kernel class (host array [4]): the instance index when the class's streamed work is a single
+ * instance, for which the FP32 packed site-projector forms (classes 1 and 3) run their single-instance
+ * kernel (instance record from the kernel arguments, neighbour lists at a fixed stride); -1 otherwise.
+ * No counterpart in the reference; used to name the kernels a benchmark's launches run. */
+int pyqed_heom_plan_class_single(const pyqed_heom_plan* plan, int32_t* inst);
 /* RK4-state elements (size in y) of the streamed work per kernel class (host array [4]): equal to
  * the element counts of plan_classes, or the packed counts with PYQED_PLAN_PACKED.  No counterpart
  * in the reference; used for the benchmark's byte model. */

@@ -25,7 +25,7 @@ SYMBOLS = [
     "pyqed_last_error", "pyqed_version", "pyqed_device_count",
     "pyqed_heom_plan_create", "pyqed_heom_plan_destroy", "pyqed_heom_plan_query", "pyqed_heom_plan_layout",
     "pyqed_heom_tables_export", "pyqed_heom_tables_size", "pyqed_heom_tables_build", "pyqed_heom_run", "pyqed_heom_rhs", "pyqed_heom_rhs_combine", "pyqed_heom_plan_set_class_tiling", "pyqed_heom_plan_get_class_tiling", "pyqed_lindblad_work_bytes", "pyqed_lindblad_run", "pyqed_heom_plan_stats",
-    "pyqed_heom_plan_classes", "pyqed_heom_plan_set_tiling", "pyqed_heom_plan_get_tiling", "pyqed_heom_plan_set_variant",
+    "pyqed_heom_plan_classes", "pyqed_heom_plan_class_single", "pyqed_heom_plan_set_tiling", "pyqed_heom_plan_get_tiling", "pyqed_heom_plan_set_variant",
     "pyqed_heom_plan_class_state", "pyqed_heom_plan_set_timing", "pyqed_heom_plan_read_timing",
     "pyqed_heom_adjoint_work_bytes", "pyqed_heom_adjoint_contract",
     "pyqed_liouville_run", "pyqed_liouville_adjoint", "pyqed_legacy_heom_run",
@@ -84,6 +84,7 @@ def lib():
                                      vp, i64, vp, i64, vp]
     L.pyqed_heom_plan_stats.argtypes = [vp, vp]
     L.pyqed_heom_plan_classes.argtypes = [vp, vp, vp]
+    L.pyqed_heom_plan_class_single.argtypes = [vp, vp]
     L.pyqed_heom_plan_set_tiling.argtypes = [vp, i64, i32]
     L.pyqed_heom_plan_get_tiling.argtypes = [vp, C.POINTER(i64), C.POINTER(i32)]
     L.pyqed_heom_plan_set_variant.argtypes = [vp, i32]

@@ -2970,6 +2970,12 @@ extern "C" int pyqed_heom_plan_get_tiling(const pyqed_heom_plan* P, int64_t* tar
   return PYQED_OK;
 }
 
+extern "C" int pyqed_heom_plan_class_single(const pyqed_heom_plan* P, int32_t* inst) {
+  if (!P || !inst) return fail(PYQED_EINVAL, "plan_class_single: null argument");
+  for (int c = 0; c < kClasses; ++c) inst[c] = P->one_inst[c];
+  return PYQED_OK;
+}
+
 extern "C" int pyqed_heom_plan_classes(const pyqed_heom_plan* P, int64_t* items, int64_t* elems) {
   if (!P) return fail(PYQED_EINVAL, "plan_classes: null plan");
   for (int c = 0; c < kClasses; ++c) {

@@ -270,6 +270,13 @@ class HeomEngine:
         _lib.check(self.L.pyqed_heom_plan_classes(self.plan, it.ctypes.data, el.ctypes.data))
         return dict(items=it.tolist(), elems=el.tolist())
 
+    def class_single(self):
+        """Per kernel class: the instance index when the class's streamed work is one instance (the FP32
+        packed projector classes 1 and 3 then run their single-instance kernel), else -1."""
+        out = np.zeros(NCLASSES, np.int32)
+        _lib.check(self.L.pyqed_heom_plan_class_single(self.plan, out.ctypes.data))
+        return out.tolist()
+
     def class_state(self):
         """RK4-state elements of the streamed work per kernel class (packed counts when packed)."""
         st = np.zeros(NCLASSES, np.int64)

@@ -147,3 +147,5 @@ def test_class_tiling_argument_checks_without_a_device():
     assert L.pyqed_heom_plan_set_class_tiling(None, 0, 1024, 8) == 1
     assert b"plan_set_class_tiling" in L.pyqed_last_error()
     assert L.pyqed_heom_plan_get_class_tiling(None, None, None) == 1
+    assert L.pyqed_heom_plan_class_single(None, None) == 1
+    assert b"plan_class_single" in L.pyqed_last_error()
