@@ -1241,9 +1241,18 @@ void heom_proj_kernel(StageParams p) {
               if (pk < np) {
                 const int ij = sIJ[pk], i = ij >> 8, j = ij & 255;
                 cpx<T> c = mk<T>(0, 0);
-                for (int kk = 0; kk < d; ++kk) {
-                  cmac(c, sH[i * rs + kk], w.X[kk * rs + j]);
-                  cmsub(c, w.X[i * rs + kk], sH[kk * rs + j]);
+                if (I.flags & F_REALH) {  // real H (H(t) too): the products with Im H are zeros, two FMAs instead of four
+                  for (int kk = 0; kk < d; ++kk) {
+                    const T hik = sH[i * rs + kk].x, hkj = sH[kk * rs + j].x;
+                    const cpx<T> xkj = w.X[kk * rs + j], xik = w.X[i * rs + kk];
+                    c.x += hik * xkj.x - xik.x * hkj;
+                    c.y += hik * xkj.y - xik.y * hkj;
+                  }
+                } else {
+                  for (int kk = 0; kk < d; ++kk) {
+                    cmac(c, sH[i * rs + kk], w.X[kk * rs + j]);
+                    cmsub(c, w.X[i * rs + kk], sH[kk * rs + j]);
+                  }
                 }
                 cpx<T> o = mul_mi(c);
                 cmsub(o, damp, w.X[i * rs + j]);
