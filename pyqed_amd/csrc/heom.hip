@@ -656,6 +656,13 @@ constexpr int kChunk = PQA_CHUNK;  // neighbour loads per lane issued together
 // the neighbour-row requests (both round trips overlap the commutator)
 #define PQA_RK4_HOIST 1
 #endif
+#ifndef PQA_SIDE_CLASSES
+// kernel classes [0, PQA_SIDE_CLASSES) run their stage launches on a second stream of the plan,
+// concurrently with the others (the classes share no elements): 1 the generic class, 2 with the VALU
+// projector class (measured slower on the 4096-instance batch: the two projector classes slow each
+// other down more than the overlap gains)
+#define PQA_SIDE_CLASSES 1
+#endif
 #ifndef PQA_HALF_COMMUTATOR
 // packed Hermitian MFMA forms: 1 form only P = H X on MFMA and complete the commutator from the
 // output tile; 0 form H X - X H on MFMA
@@ -3287,8 +3294,9 @@ int run_typed(pyqed_heom_plan* P, double2* y, char* ws, const double* field, int
       const int rc = launch(heom_unpack_kernel<T>, std::max(1, std::min(P->n_items_stream, P->num_cu * 8)), 0, st, u);
       if (rc) return rc;
     }
-    // the generic class on its own stream when other classes share the step
-    const bool split = P->cls_begin[1] > P->cls_begin[0] && P->cls_begin[kClasses] > P->cls_begin[1];
+    // the small classes on their own stream when the other classes have work too
+    constexpr int cs = PQA_SIDE_CLASSES;
+    const bool split = P->cls_begin[cs] > P->cls_begin[0] && P->cls_begin[kClasses] > P->cls_begin[cs];
     if (split) {
       PQA_HIP(hipEventRecord(P->ev_fork0, st));
       PQA_HIP(hipStreamWaitEvent(P->side0, P->ev_fork0, 0));
@@ -3299,8 +3307,8 @@ int run_typed(pyqed_heom_plan* P, double2* y, char* ws, const double* field, int
       const int64_t slot = save ? (s + 1) / save_every - 1 : -1;
       int rc;
       if (split) {
-        if ((rc = launch_step<T>(P, p, x1, x0, save, slot, P->side0, 0, 1))) return rc;
-        if ((rc = launch_step<T>(P, p, x1, x0, save, slot, st, 1, kClasses))) return rc;
+        if ((rc = launch_step<T>(P, p, x1, x0, save, slot, P->side0, 0, cs))) return rc;
+        if ((rc = launch_step<T>(P, p, x1, x0, save, slot, st, cs, kClasses))) return rc;
       } else if ((rc = launch_step<T>(P, p, x1, x0, save, slot, st, 0, kClasses))) {
         return rc;
       }
