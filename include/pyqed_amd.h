@@ -192,9 +192,10 @@ int pyqed_heom_plan_set_variant(pyqed_heom_plan* plan, int32_t variant);
  * 3 the same on 2 x 2 tiles (17 <= d <= 32)): items and elements
  * (host arrays [4], either may be NULL). */
 int pyqed_heom_plan_classes(const pyqed_heom_plan* plan, int64_t* items, int64_t* elems);
-/* Per kernel class (host array [4]): the instance index when the class's streamed work is a single
- * instance, for which the FP32 packed site-projector forms (classes 1 and 3) run their single-instance
- * kernel (instance record from the kernel arguments, neighbour lists at a fixed stride); -1 otherwise.
+/* Per kernel class (host array [4]): for classes 1 and 3 of a packed plan whose streamed work is a
+ * single instance, that instance's index (their FP32 forms then run the single-instance kernel:
+ * instance record from the kernel arguments, neighbour lists at a fixed stride); -1 otherwise, and
+ * always for classes 0 and 2.
  * No counterpart in the reference; used to name the kernels a benchmark's launches run. */
 int pyqed_heom_plan_class_single(const pyqed_heom_plan* plan, int32_t* inst);
 /* RK4-state elements (size in y) of the streamed work per kernel class (host array [4]): equal to

@@ -2831,9 +2831,10 @@ extern "C" int pyqed_heom_plan_create(const pyqed_heom_instance* in, int32_t n_i
     destroy_plan(P);
     return fail(PYQED_EUNSUPPORTED, "instance too large for the streamed kernel's LDS budget");
   }
-  // ---- projector classes with a single streamed instance: neighbour lists at a fixed stride
+  // ---- projector classes with a single streamed instance: neighbour lists at a fixed stride, for the
+  // classes whose packed forms have a single-instance kernel (the VALU class and the wide class)
   std::vector<std::vector<int2>> entfix(kClasses);
-  for (int c = 1; c < kClasses; ++c) {
+  for (int c = 1; c < kClasses && P->packed; c += 2) {
     int only = -1, count = 0;
     for (int i = 0; i < n_inst; ++i)
       if (!is_res[i] && P->inst[i].path == c) only = i, ++count;

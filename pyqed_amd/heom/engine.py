@@ -271,8 +271,8 @@ class HeomEngine:
         return dict(items=it.tolist(), elems=el.tolist())
 
     def class_single(self):
-        """Per kernel class: the instance index when the class's streamed work is one instance (the FP32
-        packed projector classes 1 and 3 then run their single-instance kernel), else -1."""
+        """Per kernel class: for classes 1 and 3 of a packed plan whose streamed work is one instance, its
+        index (their FP32 forms then run the single-instance kernel), else -1."""
         out = np.zeros(NCLASSES, np.int32)
         _lib.check(self.L.pyqed_heom_plan_class_single(self.plan, out.ctypes.data))
         return out.tolist()
