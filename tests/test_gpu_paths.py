@@ -196,3 +196,26 @@ def test_hermitian_mode_conserves_trace_like_exact_path(d):
         eng.advance(1, 0.05, "fp64")
         traces[herm] = np.trace(eng.hierarchy(0)[0])
     assert abs(traces[True] - traces[False]) <= 1e-13, traces
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_neighbour_entry_limit(prec):
+    """K + min(K, L) = 64 neighbour entries per ADO is the projector path's limit (d = 31, K = 62,
+    L = 2: the wide class at its largest row buffer); one site more (d = 32, K = 64, L = 2: 66 entries)
+    takes the generic kernel.  Both match the oracle's RK4 step from a random full hierarchy state."""
+    from pyqed_amd.heom.engine import HeomEngine
+    tol = 1e-12 if prec == "fp64" else 1e-5
+    for d, L, want in ((31, 2, 3), (32, 2, 0)):
+        inst = _instance(d, L, seed=900 + d, weight=0.7)
+        eng = HeomEngine([inst], resident=False, hermitian=False)
+        items = eng.classes()["items"]
+        assert items[want] > 0 and sum(items) == items[want], (d, items)
+        rng = np.random.default_rng(d)
+        nmax = int(eng.nmax[0])
+        state = (rng.normal(size=(nmax, d, d)) + 1j * rng.normal(size=(nmax, d, d))) * 0.1
+        ref = _oracle_step(inst, state, 0.05)
+        eng.set_state([state])
+        eng.advance(1, 0.05, prec)
+        err = np.abs(eng.hierarchy(0) - ref).max() / np.abs(ref).max()
+        print(f"d={d} L={L} {prec} classes={items} err={err:.2e}")
+        assert err <= tol
