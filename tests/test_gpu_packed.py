@@ -161,3 +161,18 @@ def test_single_instance_form_matches_table_form(d, L):
     ra = one.run(0.01, 20, "mixed", save_every=5)[0]
     rb = two.run(0.01, 20, "mixed", save_every=5)[0]
     assert np.array_equal(ra, rb)
+
+
+def test_packed_wide_form_at_the_entry_limit():
+    """The packed wide MFMA form (single-instance kernel, neighbour rows by LDS DMA) at the projector
+    path's 64-entry limit (d = 31, K = 62, depth 2): 20 mixed-precision steps against FP64."""
+    from pyqed_amd.heom.engine import HeomEngine
+    inst = _inst(31, 2, seed=931)
+    eng = HeomEngine([inst], resident=False)
+    assert eng.packed and eng.classes()["items"][3] > 0 and eng.class_single()[3] == 0
+    mix = eng.run(0.01, 20, "mixed", save_every=5)[0]
+    f64 = HeomEngine([inst], resident=False).run(0.01, 20, "fp64", save_every=5)[0]
+    err = np.abs(mix - f64).max()
+    tr = np.abs(np.trace(mix, axis1=1, axis2=2) - 1).max()
+    print(f"d=31 depth 2 packed mixed vs fp64 {err:.2e}, trace {tr:.2e}")
+    assert err <= 1e-6 and tr <= 1e-6
