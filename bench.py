@@ -1351,8 +1351,8 @@ def main():
     if args.n_inst is None:
         args.n_inst = 65536 if legacy else 4096
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: the propagator has no multi-GPU path "
-              "(see DESIGN.md for how the batch would shard).", file=sys.stderr)
+        print("bench.py runs on one GPU only: the propagator has no multi-GPU path (the instances of a "
+              "batch are independent and would split over GPUs by a size-balanced partition).", file=sys.stderr)
         sys.exit(2)
     import torch
     from pyqed_amd.heom.engine import HeomEngine
