@@ -1248,7 +1248,9 @@ void heom_proj_kernel(StageParams p) {
               if (pk < np) {
                 const int ij = sIJ[pk], i = ij >> 8, j = ij & 255;
                 cpx<T> c = mk<T>(0, 0);
-                if (I.flags & F_REALH) {  // real H (H(t) too): the products with Im H are zeros, two FMAs instead of four
+                // real H (H(t) too): the products with Im H are zeros, two FMAs instead of four (not in
+                // stage 0, whose form the second loop would take from 5 to 4 waves per SIMD)
+                if (STAGE >= 1 && (I.flags & F_REALH)) {
                   for (int kk = 0; kk < d; ++kk) {
                     const T hik = sH[i * rs + kk].x, hkj = sH[kk * rs + j].x;
                     const cpx<T> xkj = w.X[kk * rs + j], xik = w.X[i * rs + kk];
