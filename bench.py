@@ -26,7 +26,9 @@ Printed JSON line:
                  the measured HBM bytes per launch from the committed counter summary when it was
                  taken at this workload, state layout and tiling
   cpu_baseline   numpy FP64 restatement (the oracle) on the host's cores (R1: all cores, vectorised;
-                 R0: one core, the reference's per-ADO loop) on a stratified sample, with the CPU model
+                 R0: one core, the reference's per-ADO loop) on a stratified sample (a single
+                 hierarchy: consecutive steps for a time budget), with the CPU model; `value` and
+                 `leg` are the faster of the two
 
 usage: python bench.py [--steps K] [--warmup W] [--workload c3|c2|c5|c1|c4|lindblad|legacy|...]
 
