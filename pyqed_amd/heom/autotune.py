@@ -3,7 +3,8 @@ persistent grid (blocks per CU) and the kernel variant of the streamed stage ker
 steps, then refines the item size and grid of each kernel class on its own launch times.
 
 Results are cached per workload signature (instances' (d, K, L), Hermitian mode, state layout,
-precision, device name) in pyqed_amd/tuning.json; HeomEngine applies a cached entry before its first step.
+precision, device architecture and CU count) in pyqed_amd/tuning.json; HeomEngine applies a cached
+entry before its first step.
 
     python -m pyqed_amd.heom.autotune --workload c3 [--out tuning.json]
 """
