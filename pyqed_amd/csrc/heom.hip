@@ -629,7 +629,7 @@ constexpr int kMaxEnt = 64;  // neighbour entries per ADO handled by the project
 #define PQA_VALU_WAVES 4  // the same for its FP32 VALU-commutator form (small d, fewer registers)
 #endif
 #ifndef PQA_MFMA_DMIN
-#define PQA_MFMA_DMIN 12  // smallest d whose projector kernel forms the commutator on MFMA
+#define PQA_MFMA_DMIN 12  // smallest d whose projector kernel forms the commutator on MFMA (8: same step time)
 #endif
 #ifndef PQA_PACK_WAVES
 #define PQA_PACK_WAVES 4  // the same for the packed-state MFMA form
@@ -673,7 +673,7 @@ constexpr int kChunk = PQA_CHUNK;  // neighbour loads per lane issued together
 #define PQA_ROWS_DMA 1
 #endif
 #ifndef PQA_ROWS_DMA_NARROW
-// the same for the packed 16 x 16 MFMA class (0: register chunks)
+// the same for the packed 16 x 16 MFMA class (0: register chunks; 1 measured the same stage times)
 #define PQA_ROWS_DMA_NARROW 0
 #endif
 #ifndef PQA_PACK_HOIST
