@@ -103,12 +103,14 @@ int pyqed_heom_plan_create(const pyqed_heom_instance* inst, int32_t n_inst, int3
                            pyqed_heom_plan** plan);
 int pyqed_heom_plan_destroy(pyqed_heom_plan* plan);
 
-/* Sizes of the packed state: total complex elements (sum nmax*d*d), total
- * ADOs, and the workspace bytes run() needs for each precision. */
+/* Sizes of the plan's state y (the instances' hierarchies one after another): total complex elements
+ * (sum nmax*d*d, or sum nmax*d(d+1)/2 upper-triangle elements with PYQED_PLAN_PACKED), total ADOs,
+ * and the workspace bytes run() needs for each precision. */
 int pyqed_heom_plan_query(const pyqed_heom_plan* plan, int64_t* n_elems, int64_t* n_ados,
                           int64_t* ws_bytes_fp64, int64_t* ws_bytes_mixed);
-/* Per instance: element offset of ADO 0 in the packed state, nmax, and the
- * offset of its d*d block inside one trajectory slot (host arrays [n_inst]). */
+/* Per instance: element offset of ADO 0 in the state y (in upper-triangle elements with
+ * PYQED_PLAN_PACKED), nmax, and the offset of its d*d block inside one trajectory slot (host arrays
+ * [n_inst]). */
 int pyqed_heom_plan_layout(const pyqed_heom_plan* plan, int64_t* state_offset, int64_t* nmax,
                            int64_t* traj_offset);
 /* Copy instance `inst`'s tables to host arrays of shape [nmax][K]: keys
