@@ -56,3 +56,16 @@ def test_more_than_one_gpu_exits_with_a_message():
     p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120, cwd=ROOT)
     assert p.returncode == 2 and "one GPU" in p.stderr and p.stdout == ""
+
+
+def test_accuracy_sample_covers_every_stratum_and_the_largest_instances(bench):
+    insts, _, _ = bench.instances("c3", 4096)
+    idx = bench.error_sample(insts, n=64, seed=11)
+    assert len(idx) == 64 and len(set(idx)) == 64 and idx == sorted(idx)
+    strata = {(insts[i]["H"].shape[0], insts[i]["L"]) for i in idx}
+    assert len(strata) == 45 == len(bench._strata(insts))
+    dmax = max(p["H"].shape[0] for p in insts)
+    lmax = max(p["L"] for p in insts if p["H"].shape[0] == dmax)
+    big = [i for i, p in enumerate(insts) if p["H"].shape[0] == dmax and p["L"] == lmax]
+    assert set(big[:4]) <= set(idx)
+    assert idx == bench.error_sample(insts, n=64, seed=11)  # seeded
